@@ -1,0 +1,303 @@
+"""Python binding of the MI355X TFHE gate-bootstrapping engine.
+
+A thin ctypes layer over ``libcufhe_amd.so`` (C ABI in include/cufhe_amd.h
+and include/tfhe_host.h). It exists for tests, benchmarks and scripting; the
+drop-in interface for cuFHEpp users is the C++ header include/cufhe_gpu.hpp.
+There is no CPU fallback: every GPU entry point runs the HIP kernels, and
+loading fails loudly when the library has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = [
+    "CufheError", "lib", "GATES", "N0", "N1", "N2", "TLWE0_WORDS", "TLWE1_WORDS",
+    "TLWE2_WORDS", "MU0", "MU1", "MU2", "secret_key_gen", "bk_gen", "ksk_gen",
+    "encrypt_bits", "phase", "decrypt_bits", "DeviceBuffer", "Stream", "Plan",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcufhe_amd.so")
+
+N0, N1, N2 = 636, 1024, 2048
+TLWE0_WORDS, TLWE1_WORDS, TLWE2_WORDS = N0 + 1, N1 + 1, N2 + 1
+MU0 = MU1 = 1 << 29
+MU2 = 1 << 61
+
+GATES = {
+    "NAND": 0, "OR": 1, "ORYN": 2, "ORNY": 3, "AND": 4, "ANDYN": 5, "ANDNY": 6,
+    "NOR": 7, "XOR": 8, "XNOR": 9, "MUX": 10, "NMUX": 11, "NOT": 12, "COPY": 13,
+}
+
+# plaintext truth tables (reference test/plain.h:10-69)
+TRUTH = {
+    "NAND": lambda a, b: 1 - (a & b),
+    "OR": lambda a, b: a | b,
+    "ORYN": lambda a, b: a | (1 - b),
+    "ORNY": lambda a, b: (1 - a) | b,
+    "AND": lambda a, b: a & b,
+    "ANDYN": lambda a, b: a & (1 - b),
+    "ANDNY": lambda a, b: (1 - a) & b,
+    "NOR": lambda a, b: 1 - (a | b),
+    "XOR": lambda a, b: a ^ b,
+    "XNOR": lambda a, b: 1 - (a ^ b),
+}
+
+
+class CufheError(RuntimeError):
+    pass
+
+
+class GateDesc(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_int32), ("out", ctypes.c_uint32), ("in0", ctypes.c_uint32),
+                ("in1", ctypes.c_uint32), ("inc", ctypes.c_uint32)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libcufhe_amd.so (built in-tree by cufhe_amd/build.py)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CufheError("%s is not built: run `python cufhe_amd/build.py`" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp, sz, i32, u32, u64 = (ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                             ctypes.c_uint32, ctypes.c_uint64)
+    sigs = {
+        "cufhe_version": ([], ctypes.c_char_p),
+        "cufhe_last_error": ([], ctypes.c_char_p),
+        "cufhe_set_gpu_num": ([i32], i32),
+        "cufhe_initialize": ([], i32),
+        "cufhe_initialize_lvl01": ([vp, sz, i32, vp, sz], i32),
+        "cufhe_initialize_lvl02": ([vp, sz, i32, vp, sz], i32),
+        "cufhe_cleanup": ([], i32),
+        "cufhe_cleanup_lvl02": ([], i32),
+        "cufhe_synchronize": ([], i32),
+        "cufhe_stream_create": ([ctypes.POINTER(vp)], i32),
+        "cufhe_stream_destroy": ([vp], i32),
+        "cufhe_stream_query": ([vp], i32),
+        "cufhe_stream_synchronize": ([vp], i32),
+        "cufhe_device_alloc": ([ctypes.POINTER(vp), sz], i32),
+        "cufhe_device_free": ([vp], i32),
+        "cufhe_host_alloc": ([ctypes.POINTER(vp), sz], i32),
+        "cufhe_host_free": ([vp], i32),
+        "cufhe_memcpy_h2d": ([vp, vp, sz, vp], i32),
+        "cufhe_memcpy_d2h": ([vp, vp, sz, vp], i32),
+        "cufhe_memset_d": ([vp, i32, sz, vp], i32),
+        "cufhe_memcpy_d2d": ([vp, vp, sz, vp], i32),
+        "cufhe_plan_jobs_per_group": ([vp], i32),
+        "cufhe_plan_run_timed": ([vp, vp, ctypes.POINTER(ctypes.c_float)], i32),
+        "cufhe_gate_lvl1": ([i32, vp, vp, vp, vp, vp], i32),
+        "cufhe_gate_lvl02": ([i32, vp, vp, vp, vp, vp], i32),
+        "cufhe_plan_create_lvl1": ([vp, i32, vp, i32, i32, ctypes.POINTER(vp)], i32),
+        "cufhe_plan_run": ([vp, vp], i32),
+        "cufhe_plan_destroy": ([vp], i32),
+        "cufhe_plan_bootstraps": ([vp], i32),
+        "cufhe_gate_batch_lvl1": ([vp, i32, vp, i32, vp], i32),
+        "cufhe_identity_key_switch_lvl10": ([vp, vp, i32, vp], i32),
+        "cufhe_bootstrap_lvl01": ([vp, vp, u32, i32, i32, vp], i32),
+        "cufhe_ffnt_forward": ([i32, vp, i32, vp, i32, vp], i32),
+        "cufhe_ffnt_inverse": ([i32, vp, vp, i32, vp], i32),
+        "cufhe_trgsw_to_fourier": ([i32, vp, vp, i32, vp], i32),
+        "cufhe_cmux_accumulate": ([i32, vp, vp, u32, vp], i32),
+        "tfhe_secret_key_gen": ([u64, i32, vp, vp, vp], i32),
+        "tfhe_bk_gen": ([i32, i32, u64, vp, vp, vp, sz, ctypes.POINTER(sz)], i32),
+        "tfhe_ksk_gen": ([i32, u64, vp, vp, i32, vp, sz], i32),
+        "tfhe_tlwe_encrypt_bits": ([i32, u64, vp, vp, sz, vp], i32),
+        "tfhe_tlwe_phase": ([i32, vp, vp, sz, vp], i32),
+        "tfhe_tlwe_decrypt_bits": ([i32, vp, vp, sz, vp], i32),
+        "tfhe_tlwe_words": ([i32], sz),
+        "tfhe_bk_words": ([i32, i32], sz),
+        "tfhe_ksk_words": ([i32], sz),
+    }
+    for name, (args, res) in sigs.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().cufhe_last_error().decode(errors="replace")
+        raise CufheError("%s failed (%d): %s" % (what, rc, msg))
+    return rc
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+# ---------------------------------------------------------------- host layer
+
+def secret_key_gen(seed, block_binary=False):
+    k0 = np.zeros(N0, np.uint32)
+    k1 = np.zeros(N1, np.uint32)
+    k2 = np.zeros(N2, np.uint64)
+    check(lib().tfhe_secret_key_gen(seed, int(block_binary), _ptr(k0), _ptr(k1), _ptr(k2)),
+          "tfhe_secret_key_gen")
+    return k0, k1, k2
+
+
+def bk_gen(level, key_bundle, seed, k0, kt):
+    words = lib().tfhe_bk_words(level, int(key_bundle))
+    out = np.zeros(words, np.uint32 if level == 1 else np.uint64)
+    check(lib().tfhe_bk_gen(level, int(key_bundle), seed, _ptr(k0), _ptr(kt), _ptr(out), words,
+                            None), "tfhe_bk_gen")
+    return out
+
+
+def ksk_gen(level, seed, kd, k0):
+    words = lib().tfhe_ksk_words(level)
+    out = np.zeros(words, np.uint32)
+    check(lib().tfhe_ksk_gen(level, seed, _ptr(kd), _ptr(k0), 0, _ptr(out), words),
+          "tfhe_ksk_gen")
+    return out
+
+
+def tlwe_words(level):
+    return {0: TLWE0_WORDS, 1: TLWE1_WORDS, 2: TLWE2_WORDS}[level]
+
+
+def encrypt_bits(level, seed, key, bits):
+    bits = np.ascontiguousarray(bits, np.uint8)
+    dt = np.uint64 if level == 2 else np.uint32
+    out = np.zeros((len(bits), tlwe_words(level)), dt)
+    check(lib().tfhe_tlwe_encrypt_bits(level, seed, _ptr(key), _ptr(bits), len(bits), _ptr(out)),
+          "tfhe_tlwe_encrypt_bits")
+    return out
+
+
+def phase(level, key, ct):
+    ct = np.ascontiguousarray(ct)
+    n = ct.shape[0] if ct.ndim == 2 else 1
+    out = np.zeros(n, np.float64)
+    check(lib().tfhe_tlwe_phase(level, _ptr(key), _ptr(ct), n, _ptr(out)), "tfhe_tlwe_phase")
+    return out
+
+
+def decrypt_bits(level, key, ct):
+    return (phase(level, key, ct) > 0).astype(np.uint8)
+
+
+# ---------------------------------------------------------------- device layer
+
+class DeviceBuffer:
+    """Device allocation owned by Python (hipMalloc through the C ABI)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        check(lib().cufhe_device_alloc(ctypes.byref(p), max(self.nbytes, 1)),
+              "cufhe_device_alloc")
+        self.ptr = p
+
+    @classmethod
+    def from_numpy(cls, a, stream=None):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a, stream)
+        return b
+
+    def upload(self, a, stream=None):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib().cufhe_memcpy_h2d(self.ptr, _ptr(a), a.nbytes, _sp(stream)), "memcpy_h2d")
+        check(lib().cufhe_stream_synchronize(_sp(stream)), "stream_synchronize")
+
+    def download(self, dtype, shape, stream=None):
+        out = np.empty(shape, dtype)
+        check(lib().cufhe_memcpy_d2h(_ptr(out), self.ptr, out.nbytes, _sp(stream)), "memcpy_d2h")
+        check(lib().cufhe_stream_synchronize(_sp(stream)), "stream_synchronize")
+        return out
+
+    def at(self, byte_offset):
+        return ctypes.c_void_p(self.ptr.value + int(byte_offset))
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            lib().cufhe_device_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        p = ctypes.c_void_p()
+        check(lib().cufhe_stream_create(ctypes.byref(p)), "cufhe_stream_create")
+        self.handle = p
+
+    def query(self):
+        rc = lib().cufhe_stream_query(self.handle)
+        if rc < 0:
+            check(rc, "cufhe_stream_query")
+        return rc == 1
+
+    def synchronize(self):
+        check(lib().cufhe_stream_synchronize(self.handle), "cufhe_stream_synchronize")
+
+    def destroy(self):
+        if self.handle is not None:
+            lib().cufhe_stream_destroy(self.handle)
+            self.handle = None
+
+
+def _sp(stream):
+    if stream is None:
+        return None
+    return stream.handle if isinstance(stream, Stream) else stream
+
+
+class Plan:
+    """Batched gates over a device pool of lvl1 TLWEs."""
+
+    def __init__(self, gates, pool, jobs_per_group=0, use_graph=False):
+        arr = (GateDesc * len(gates))()
+        for i, g in enumerate(gates):
+            op, out, in0, in1, inc = g
+            arr[i] = GateDesc(GATES[op] if isinstance(op, str) else int(op), out, in0, in1, inc)
+        self._arr = arr
+        h = ctypes.c_void_p()
+        check(lib().cufhe_plan_create_lvl1(ctypes.cast(arr, ctypes.c_void_p), len(gates),
+                                           pool.ptr, int(jobs_per_group), int(use_graph),
+                                           ctypes.byref(h)), "cufhe_plan_create_lvl1")
+        self.handle = h
+
+    @property
+    def bootstraps(self):
+        return lib().cufhe_plan_bootstraps(self.handle)
+
+    @property
+    def jobs_per_group(self):
+        return lib().cufhe_plan_jobs_per_group(self.handle)
+
+    def run(self, stream=None):
+        check(lib().cufhe_plan_run(self.handle, _sp(stream)), "cufhe_plan_run")
+
+    def run_timed(self, stream=None):
+        """Eager run with HIP events around each kernel; returns ms of
+        (key switching, blind rotation, combine)."""
+        ms = (ctypes.c_float * 3)()
+        check(lib().cufhe_plan_run_timed(self.handle, _sp(stream), ms), "cufhe_plan_run_timed")
+        return float(ms[0]), float(ms[1]), float(ms[2])
+
+    def destroy(self):
+        if self.handle is not None:
+            lib().cufhe_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

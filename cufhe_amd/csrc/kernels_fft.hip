@@ -1,0 +1,435 @@
+// FFT-backend kernels: key conversion, batched blind rotation, plain
+// per-gate kernels, key switching, MUX combine, and FFNT entry points.
+#include "br_fft.hpp"
+#include "keyswitch.hpp"
+#include "launch.hpp"
+
+namespace cufhe_amd {
+
+constexpr int kTlwe0Pad = 640;  // lvl0 TLWE (637 words) padded in LDS
+
+// ---------------------------------------------------------------------------
+// Polynomial -> Fourier spectra. One wave per polynomial.
+// Reference src/bootstrap_gpu.cu:59-112 (__TRGSW2FFT__): signed torus value
+// * 2^-w, fold, twist, forward FFT. INTEGER=true skips the 2^-w scaling and
+// reads int32 inputs (used for X^a - 1 and test polynomials).
+template <class P, bool INTEGER>
+__global__ __launch_bounds__(256) void k_to_fourier(cplx* out, const void* in, int npolys,
+                                                    FourierKeys<P> K)
+{
+    using T = typename P::T;
+    using ST = std::make_signed_t<T>;
+    constexpr int N = P::n, H = N / 2, R = H / 64;
+    __shared__ cplx sc[4][H];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int poly = blockIdx.x * 4 + wave;
+    if (poly >= npolys) return;  // no workgroup barriers below
+    cplx z[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int m = lane + 64 * r;
+        double re, im;
+        if constexpr (INTEGER) {
+            const int32_t* p = static_cast<const int32_t*>(in) + static_cast<size_t>(poly) * N;
+            re = static_cast<double>(p[m]);
+            im = static_cast<double>(p[m + H]);
+        }
+        else {
+            const T* p = static_cast<const T*>(in) + static_cast<size_t>(poly) * N;
+            constexpr double s = 1.0 / kTorusScale<T>;
+            re = static_cast<double>(static_cast<ST>(p[m])) * s;
+            im = static_cast<double>(static_cast<ST>(p[m + H])) * s;
+        }
+        z[r] = cplx{re, im};
+    }
+    fft_forward<H>(z, sc[wave], K.fft);
+#pragma unroll
+    for (int d = 0; d < R; d++) out[static_cast<size_t>(poly) * H + d * 64 + lane] = z[d];
+}
+
+// X^a - 1 polynomials for a in [0, 2N) (reference src/ntt_small_modulus.cu
+// :928-981 XaittGen): coefficient 0 is -1, coefficient a mod N gets +1
+// (a < N) or -1 (a >= N).
+template <class P>
+__global__ void k_xai_poly(int32_t* out)
+{
+    constexpr int N = P::n;
+    const int a = blockIdx.x;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        int32_t v = (i == 0) ? -1 : 0;
+        if (i == (a & (N - 1))) v += (a < N) ? 1 : -1;
+        out[static_cast<size_t>(a) * N + i] = v;
+    }
+}
+
+// Spectra -> real polynomial (inverse FFT, untwist incl. 1/H), as doubles.
+template <class P>
+__global__ __launch_bounds__(256) void k_from_fourier(double* out, const cplx* in, int npolys,
+                                                      FourierKeys<P> K)
+{
+    constexpr int N = P::n, H = N / 2, R = H / 64;
+    __shared__ cplx sc[4][H];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int poly = blockIdx.x * 4 + wave;
+    if (poly >= npolys) return;
+    cplx z[R];
+#pragma unroll
+    for (int d = 0; d < R; d++) z[d] = in[static_cast<size_t>(poly) * H + d * 64 + lane];
+    fft_inverse<H>(z, sc[wave], K.fft);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int m = lane + 64 * r;
+        const cplx v = cscale(z[r], 1.0 / H);
+        out[static_cast<size_t>(poly) * N + m] = v.x;
+        out[static_cast<size_t>(poly) * N + m + H] = v.y;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched blind rotation: C jobs per workgroup, two waves per job.
+template <class P, bool KB, int C>
+__global__ __launch_bounds__(128 * C, 3) void k_br_fft(const BrJob* __restrict__ jobs, int njobs,
+                                                     typename P::T mu, FourierKeys<P> K)
+{
+    using T = typename P::T;
+    constexpr int N = P::n, H = N / 2, n0 = lvl0param::n;
+    constexpr bool kL = AccStore<P>::kLds;
+    constexpr int NB = BrShape<P>::NB;
+    __shared__ cplx sh_spec[2 * C][NB * H];
+    __shared__ uint32_t sh_tl0[C][kTlwe0Pad];
+    __shared__ T sh_coef[kL ? 2 * C : 1][kL ? N : 1];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int slot = wave >> 1, j = wave & 1;
+    const int raw = blockIdx.x * C + slot;
+    const bool live = raw < njobs;
+    const BrJob jb = jobs[live ? raw : njobs - 1];
+    for (int i = threadIdx.x & 127; i <= n0; i += 128) {
+        uint32_t v = static_cast<uint32_t>(jb.ca) * jb.in0[i];
+        if (jb.in1) v += static_cast<uint32_t>(jb.cb) * jb.in1[i];
+        if (i == n0) v += jb.offset;
+        sh_tl0[slot][i] = v;
+    }
+    __syncthreads();
+    AccStore<P> acc;
+    if constexpr (kL) acc.lds = sh_coef[wave];
+    blind_rotate_fft<P, KB>(sh_tl0[slot], mu, acc, j, sh_spec[wave], sh_spec[wave ^ 1], K);
+    if (live) sample_extract_store<P>(acc, j, static_cast<T*>(jb.out));
+}
+
+// ---------------------------------------------------------------------------
+// Plain per-gate form, lvl1 API (IKS-BR order), one gate per launch.
+// Binary gates: one job; MUX/NMUX: two jobs run concurrently, then combined
+// (reference src/bootstrap_gpu.cu:1223-1246, :2069-2197).
+template <bool KB, bool MUX>
+__global__ __launch_bounds__(MUX ? 256 : 128, 2) void k_gate_lvl1_fft(
+    uint32_t* out, const uint32_t* in0, const uint32_t* in1, const uint32_t* inc, int32_t ca,
+    int32_t cb, uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
+    FourierKeys<lvl1param> K)
+{
+    using P = lvl1param;
+    constexpr int C = MUX ? 2 : 1;
+    constexpr int N = P::n, H = N / 2;
+    constexpr uint32_t mu = static_cast<uint32_t>(P::μ);
+    constexpr int NB = BrShape<P>::NB;
+    __shared__ cplx sh_spec[2 * C][NB * H];
+    __shared__ uint32_t sh_tl0[C][kTlwe0Pad];
+    __shared__ uint32_t sh_res[MUX ? 2 : 1][MUX ? N + 1 : 1];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int slot = wave >> 1, j = wave & 1;
+    const int t = threadIdx.x & 127;
+    if constexpr (MUX) {
+        // slot 0: c + t - mu ; slot 1: -c + f - mu (both AND-like, +-mu out)
+        const uint32_t* b = slot == 0 ? in1 : in0;
+        key_switch_columns<TFHEpp::lvl10param, false, 5>(sh_tl0[slot], inc, b,
+                                                         slot == 0 ? 1 : -1, 1, 0u - mu, ksk,
+                                                         t, 128);
+    }
+    else {
+        key_switch_columns<TFHEpp::lvl10param, false, 5>(sh_tl0[0], in0, in1, ca, cb, offset,
+                                                         ksk, t, 128);
+    }
+    __syncthreads();
+    AccStore<P> acc;
+    blind_rotate_fft<P, KB>(sh_tl0[slot], mu, acc, j, sh_spec[wave], sh_spec[wave ^ 1], K);
+    if constexpr (!MUX) {
+        sample_extract_store<P>(acc, j, out);
+    }
+    else {
+        sample_extract_store<P>(acc, j, sh_res[slot]);
+        __syncthreads();
+        for (int i = threadIdx.x; i <= N; i += 256) {
+            uint32_t v = sh_res[0][i] + sh_res[1][i];
+            if (i == N) v += mu;
+            out[i] = mux_sign > 0 ? v : 0u - v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain per-gate form, lvl02 API (BR-IKS order): lvl0 in, BR to lvl2
+// (Torus64), SE, key switch lvl2 -> lvl0 (reference src/bootstrap_gpu.cu
+// :1248-1273, :1305-1331, MUX :1949-2060). MUX runs its two rotations one
+// after the other in the same wave pair (LDS holds one lvl2 job).
+template <bool KB, bool MUX>
+__global__ __launch_bounds__(128) void k_gate_lvl02_fft(
+    uint32_t* out, const uint32_t* in0, const uint32_t* in1, const uint32_t* inc, int32_t ca,
+    int32_t cb, uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
+    FourierKeys<lvl2param> K)
+{
+    using P = lvl2param;
+    using T = uint64_t;
+    constexpr int N = P::n, H = N / 2, n0 = lvl0param::n;
+    constexpr T mu = static_cast<T>(P::μ);
+    constexpr int NB = BrShape<P>::NB;
+    __shared__ cplx sh_spec[2][NB * H];
+    __shared__ T sh_coef[2][N];
+    __shared__ uint32_t sh_tl0[kTlwe0Pad];
+    __shared__ T sh_res[MUX ? 2 : 1][N + 8];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int j = wave & 1;
+    const int t = threadIdx.x;
+    AccStore<P> acc;
+    acc.lds = sh_coef[wave];
+#pragma unroll 1
+    for (int pass = 0; pass < (MUX ? 2 : 1); pass++) {
+        // MUX: pass 0 = c + t - mu, pass 1 = -c + f - mu
+        const uint32_t* a = MUX ? inc : in0;
+        const uint32_t* b = MUX ? (pass == 0 ? in1 : in0) : in1;
+        const int32_t sa = MUX ? (pass == 0 ? 1 : -1) : ca;
+        const int32_t sb = MUX ? 1 : cb;
+        const uint32_t off = MUX ? 0u - static_cast<uint32_t>(lvl0param::μ) : offset;
+        __syncthreads();
+        for (int i = t; i <= n0; i += 128) {
+            uint32_t v = static_cast<uint32_t>(sa) * a[i] + static_cast<uint32_t>(sb) * b[i];
+            if (i == n0) v += off;
+            sh_tl0[i] = v;
+        }
+        __syncthreads();
+        blind_rotate_fft<P, KB>(sh_tl0, mu, acc, j, sh_spec[wave], sh_spec[wave ^ 1], K);
+        sample_extract_store<P>(acc, j, sh_res[pass]);
+    }
+    __syncthreads();
+    if constexpr (MUX) {
+        key_switch_columns<TFHEpp::lvl20param, false, 5>(out, sh_res[0], sh_res[1], mux_sign,
+                                                         mux_sign, mux_sign > 0 ? mu : 0 - mu,
+                                                         ksk, t, 128);
+    }
+    else {
+        key_switch_columns<TFHEpp::lvl20param, false, 5>(out, sh_res[0], nullptr, 1, 0, 0, ksk, t,
+                                                         128);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched key switching: one wave per job, lanes over output columns.
+template <class KS, bool SUBSET>
+__global__ __launch_bounds__(256) void k_ks_batch(const KsJob* __restrict__ jobs, int njobs,
+                                                  const uint32_t* ksk)
+{
+    using DT = typename KS::domainP::T;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int job = blockIdx.x * 4 + wave;
+    if (job >= njobs) return;
+    const KsJob jb = jobs[job];
+    key_switch_columns<KS, SUBSET, 10>(jb.out, static_cast<const DT*>(jb.in0),
+                                       static_cast<const DT*>(jb.in1), jb.ca, jb.cb,
+                                       static_cast<DT>(jb.offset), ksk, threadIdx.x & 63, 64);
+}
+
+// Elementwise TLWE combinations (MUX/NMUX combine, NOT, COPY).
+__global__ __launch_bounds__(256) void k_linear(const LinJob* __restrict__ jobs, int njobs)
+{
+    const LinJob jb = jobs[blockIdx.x];
+    for (int i = threadIdx.x; i < jb.words; i += 256) {
+        uint32_t v = static_cast<uint32_t>(jb.s1) * jb.r1[i];
+        if (jb.r0) v += static_cast<uint32_t>(jb.s0) * jb.r0[i];
+        if (i == jb.words - 1) v += jb.offset;
+        jb.out[i] = v;
+    }
+}
+
+// One CMUX accumulate step on a TRLWE in global memory (reference
+// include/gatebootstrapping_gpu.cuh:69-221): acc += ExtProd((X^a-1) acc, C).
+template <class P>
+__global__ __launch_bounds__(128, 2) void k_cmux_fft(typename P::T* trlwe, const cplx* trgsw,
+                                                   uint32_t abar, FourierKeys<P> K)
+{
+    using T = typename P::T;
+    constexpr int N = P::n, H = N / 2, R = N / 128;
+    constexpr bool kL = AccStore<P>::kLds;
+    constexpr int NB = BrShape<P>::NB;
+    __shared__ cplx sh_spec[2][NB * H];
+    __shared__ T sh_coef[kL ? 2 : 1][kL ? N : 1];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    AccStore<P> acc;
+    if constexpr (kL) acc.lds = sh_coef[wave];
+#pragma unroll
+    for (int r = 0; r < 2 * R; r++)
+        acc.set(r, lane, trlwe[wave * N + AccStore<P>::coef_index(r, lane)]);
+    if constexpr (kL) wave_lds_sync();
+    br_step_fft<P, false>(acc, wave, abar, 0, trgsw, sh_spec[wave], sh_spec[wave ^ 1], K);
+#pragma unroll
+    for (int r = 0; r < 2 * R; r++)
+        trlwe[wave * N + AccStore<P>::coef_index(r, lane)] = acc.get(r, lane);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+
+template <class P>
+void launch_to_fourier(cplx* out, const void* in, int npolys, bool integer,
+                       const FourierKeys<P>& K, hipStream_t st)
+{
+    const dim3 grid((npolys + 3) / 4), block(256);
+    if (integer)
+        hipLaunchKernelGGL((k_to_fourier<P, true>), grid, block, 0, st, out, in, npolys, K);
+    else
+        hipLaunchKernelGGL((k_to_fourier<P, false>), grid, block, 0, st, out, in, npolys, K);
+}
+template void launch_to_fourier<lvl1param>(cplx*, const void*, int, bool,
+                                           const FourierKeys<lvl1param>&, hipStream_t);
+template void launch_to_fourier<lvl2param>(cplx*, const void*, int, bool,
+                                           const FourierKeys<lvl2param>&, hipStream_t);
+
+template <class P>
+void launch_from_fourier(double* out, const cplx* in, int npolys, const FourierKeys<P>& K,
+                         hipStream_t st)
+{
+    hipLaunchKernelGGL((k_from_fourier<P>), dim3((npolys + 3) / 4), dim3(256), 0, st, out, in,
+                       npolys, K);
+}
+template void launch_from_fourier<lvl1param>(double*, const cplx*, int,
+                                             const FourierKeys<lvl1param>&, hipStream_t);
+template void launch_from_fourier<lvl2param>(double*, const cplx*, int,
+                                             const FourierKeys<lvl2param>&, hipStream_t);
+
+template <class P>
+void launch_xai_poly(int32_t* out, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_xai_poly<P>), dim3(2 * P::n), dim3(256), 0, st, out);
+}
+template void launch_xai_poly<lvl1param>(int32_t*, hipStream_t);
+template void launch_xai_poly<lvl2param>(int32_t*, hipStream_t);
+
+template <class P, bool KB>
+void launch_br_fft(const BrJob* jobs, int njobs, int c, typename P::T mu,
+                   const FourierKeys<P>& K, hipStream_t st)
+{
+    const dim3 block(128 * c);
+    const dim3 grid((njobs + c - 1) / c);
+    switch (c) {
+        case 1:
+            hipLaunchKernelGGL((k_br_fft<P, KB, 1>), grid, block, 0, st, jobs, njobs, mu, K);
+            break;
+        case 2:
+            if constexpr (sizeof(typename P::T) == 4)
+                hipLaunchKernelGGL((k_br_fft<P, KB, 2>), grid, block, 0, st, jobs, njobs, mu,
+                                   K);
+            break;
+        case 4:
+            if constexpr (sizeof(typename P::T) == 4 && KB)
+                hipLaunchKernelGGL((k_br_fft<P, KB, 4>), grid, block, 0, st, jobs, njobs, mu,
+                                   K);
+            break;
+        case 8:
+            break;
+        default:
+            break;
+    }
+}
+template void launch_br_fft<lvl1param, true>(const BrJob*, int, int, uint32_t,
+                                             const FourierKeys<lvl1param>&, hipStream_t);
+template void launch_br_fft<lvl1param, false>(const BrJob*, int, int, uint32_t,
+                                              const FourierKeys<lvl1param>&, hipStream_t);
+template void launch_br_fft<lvl2param, true>(const BrJob*, int, int, uint64_t,
+                                             const FourierKeys<lvl2param>&, hipStream_t);
+template void launch_br_fft<lvl2param, false>(const BrJob*, int, int, uint64_t,
+                                              const FourierKeys<lvl2param>&, hipStream_t);
+
+void launch_gate_lvl1_fft(bool kb, bool mux, uint32_t* out, const uint32_t* in0,
+                          const uint32_t* in1, const uint32_t* inc, int32_t ca, int32_t cb,
+                          uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
+                          const FourierKeys<lvl1param>& K, hipStream_t st)
+{
+    const dim3 grid(1), block(mux ? 256 : 128);
+#define CUFHE_L(KBV, MUXV)                                                                    \
+    hipLaunchKernelGGL((k_gate_lvl1_fft<KBV, MUXV>), grid, block, 0, st, out, in0, in1, inc, \
+                       ca, cb, offset, mux_sign, ksk, K)
+    if (kb) {
+        if (mux) CUFHE_L(true, true); else CUFHE_L(true, false);
+    }
+    else {
+        if (mux) CUFHE_L(false, true); else CUFHE_L(false, false);
+    }
+#undef CUFHE_L
+}
+
+void launch_gate_lvl02_fft(bool kb, bool mux, uint32_t* out, const uint32_t* in0,
+                           const uint32_t* in1, const uint32_t* inc, int32_t ca, int32_t cb,
+                           uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
+                           const FourierKeys<lvl2param>& K, hipStream_t st)
+{
+    const dim3 grid(1), block(128);
+#define CUFHE_L(KBV, MUXV)                                                                     \
+    hipLaunchKernelGGL((k_gate_lvl02_fft<KBV, MUXV>), grid, block, 0, st, out, in0, in1, inc, \
+                       ca, cb, offset, mux_sign, ksk, K)
+    if (kb) {
+        if (mux) CUFHE_L(true, true); else CUFHE_L(true, false);
+    }
+    else {
+        if (mux) CUFHE_L(false, true); else CUFHE_L(false, false);
+    }
+#undef CUFHE_L
+}
+
+void launch_ks_batch(int domain_level, bool subset, const KsJob* jobs, int njobs,
+                     const uint32_t* ksk, hipStream_t st)
+{
+    const dim3 grid((njobs + 3) / 4), block(256);
+    if (domain_level == 1) {
+        if (subset)
+            hipLaunchKernelGGL((k_ks_batch<TFHEpp::lvl10param, true>), grid, block, 0, st, jobs,
+                               njobs, ksk);
+        else
+            hipLaunchKernelGGL((k_ks_batch<TFHEpp::lvl10param, false>), grid, block, 0, st,
+                               jobs, njobs, ksk);
+    }
+    else {
+        hipLaunchKernelGGL((k_ks_batch<TFHEpp::lvl20param, false>), grid, block, 0, st, jobs,
+                           njobs, ksk);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_linear_one(LinJob jb)
+{
+    for (int i = threadIdx.x; i < jb.words; i += 256) {
+        uint32_t v = static_cast<uint32_t>(jb.s1) * jb.r1[i];
+        if (jb.r0) v += static_cast<uint32_t>(jb.s0) * jb.r0[i];
+        if (i == jb.words - 1) v += jb.offset;
+        jb.out[i] = v;
+    }
+}
+
+void launch_linear_one(const LinJob& jb, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_linear_one, dim3(1), dim3(256), 0, st, jb);
+}
+
+void launch_linear(const LinJob* jobs, int njobs, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_linear, dim3(njobs), dim3(256), 0, st, jobs, njobs);
+}
+
+template <class P>
+void launch_cmux_fft(typename P::T* trlwe, const cplx* trgsw, uint32_t abar,
+                     const FourierKeys<P>& K, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_cmux_fft<P>), dim3(1), dim3(128), 0, st, trlwe, trgsw, abar, K);
+}
+template void launch_cmux_fft<lvl1param>(uint32_t*, const cplx*, uint32_t,
+                                         const FourierKeys<lvl1param>&, hipStream_t);
+template void launch_cmux_fft<lvl2param>(uint64_t*, const cplx*, uint32_t,
+                                         const FourierKeys<lvl2param>&, hipStream_t);
+
+}  // namespace cufhe_amd

@@ -1,0 +1,113 @@
+// Identity key switching lvl1 -> lvl0 and lvl2 -> lvl0 with the gate's
+// linear combination fused in front.
+//
+// Reference: include/keyswitch_gpu.cuh:13-23 (iksoffsetgen), :33-50
+// (KeySwitchTorusConvert), :128-187 (KeySwitchFromTLWE), :189-256
+// (IdentityKeySwitchPreAdd). For each output coefficient i in [0, n0]:
+//   res = (i == n0) ? conv(ca*in0[dn] + cb*in1[dn] + offset)
+//                   : (subset && i < n0 ? conv(ca*in0[i] + cb*in1[i]) : 0)
+//   for each switched input coefficient j (all, or j >= n0 when subset):
+//     tmp = ca*in0[j] + cb*in1[j] + decomp_offset + round_offset
+//     for each of t signed digits v of basebit bits (most significant first):
+//       res -= sign(v) * KSK[j'][digit][|v|-1][i]      (j' = j - first)
+// KSK layout: [switched j][t][2^(basebit-1)][n0+1] uint32.
+#pragma once
+
+#include "device_common.hpp"
+
+namespace cufhe_amd {
+
+template <class KS>
+struct KsTraits {
+    using DT = typename KS::domainP::T;
+    static constexpr int W = kTorusBits<DT>;
+    static constexpr uint32_t dn = KS::domainP::k * KS::domainP::n;
+    static constexpr uint32_t n0 = KS::targetP::k * KS::targetP::n;
+    static constexpr uint32_t numbase = 1u << (KS::basebit - 1);
+    static constexpr DT decomp_offset()
+    {
+        DT o = 0;
+        for (uint32_t i = 1; i <= KS::t; i++)
+            o += static_cast<DT>(numbase) << (W - i * KS::basebit);
+        return o;
+    }
+    static constexpr DT kRound =
+        (KS::basebit * KS::t) < static_cast<uint32_t>(W)
+            ? static_cast<DT>(1) << (W - (1 + KS::basebit * KS::t))
+            : 0;
+    static constexpr DT kOffset = decomp_offset() + kRound;
+    __device__ __forceinline__ static uint32_t convert(DT v)
+    {
+        if constexpr (W == 32)
+            return static_cast<uint32_t>(v);
+        else
+            return static_cast<uint32_t>((v + (static_cast<DT>(1) << 31)) >> 32);
+    }
+    static constexpr size_t row_words = n0 + 1;
+    static constexpr size_t digit_words = numbase * row_words;
+    static constexpr size_t coef_words = KS::t * digit_words;
+};
+
+// Key switch of the combination ca*in0 + cb*in1 (+offset on the body) for
+// the output columns i = col0, col0 + stride, ... (< n0 + 1) of one job.
+// All threads of a wave see the same j and digit sequence, so branches are
+// wave-uniform and each KSK row read is a coalesced run over columns.
+template <class KS, bool SUBSET, int MAXCOLS>
+__device__ __forceinline__ void key_switch_columns(uint32_t* out, const typename KS::domainP::T* in0,
+                                                   const typename KS::domainP::T* in1, int32_t ca,
+                                                   int32_t cb, typename KS::domainP::T offset,
+                                                   const uint32_t* ksk, int col0, int stride)
+{
+    using TR = KsTraits<KS>;
+    using DT = typename TR::DT;
+    constexpr uint32_t first = SUBSET ? TR::n0 : 0;
+    const DT ua = static_cast<DT>(static_cast<std::make_signed_t<DT>>(ca));
+    const DT ub = static_cast<DT>(static_cast<std::make_signed_t<DT>>(cb));
+    uint32_t res[MAXCOLS];
+#pragma unroll
+    for (int c = 0; c < MAXCOLS; c++) {
+        const int i = col0 + c * stride;
+        uint32_t r = 0;
+        if (i == static_cast<int>(TR::n0)) {
+            DT v = ua * in0[TR::dn] + offset;
+            if (in1) v += ub * in1[TR::dn];
+            r = TR::convert(v);
+        }
+        else if (SUBSET && i < static_cast<int>(TR::n0)) {
+            DT v = ua * in0[i];
+            if (in1) v += ub * in1[i];
+            r = TR::convert(v);
+        }
+        res[c] = r;
+    }
+#pragma unroll 2
+    for (uint32_t j = first; j < TR::dn; j++) {
+        DT tmp = ua * in0[j] + TR::kOffset;
+        if (in1) tmp += ub * in1[j];
+        const uint32_t* kj = ksk + static_cast<size_t>(j - first) * TR::coef_words;
+#pragma unroll
+        for (uint32_t k = 0; k < KS::t; k++) {
+            const int32_t val =
+                static_cast<int32_t>((tmp >> (TR::W - (k + 1) * KS::basebit)) &
+                                     ((static_cast<DT>(1) << KS::basebit) - 1)) -
+                static_cast<int32_t>(TR::numbase);
+            if (val == 0) continue;
+            const uint32_t* row = kj + k * TR::digit_words + (abs(val) - 1) * TR::row_words;
+#pragma unroll
+            for (int c = 0; c < MAXCOLS; c++) {
+                const int i = col0 + c * stride;
+                if (i <= static_cast<int>(TR::n0)) {
+                    const uint32_t x = row[i];
+                    res[c] = val > 0 ? res[c] - x : res[c] + x;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < MAXCOLS; c++) {
+        const int i = col0 + c * stride;
+        if (i <= static_cast<int>(TR::n0)) out[i] = res[c];
+    }
+}
+
+}  // namespace cufhe_amd

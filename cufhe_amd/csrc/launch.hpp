@@ -1,0 +1,45 @@
+// Host-side launch wrappers of the HIP kernels (defined in kernels_*.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_common.hpp"
+
+namespace cufhe_amd {
+
+struct cplx;
+template <class P>
+struct FourierKeys;
+template <class P>
+struct NttKeys;
+
+template <class P>
+void launch_to_fourier(cplx* out, const void* in, int npolys, bool integer,
+                       const FourierKeys<P>& K, hipStream_t st);
+template <class P>
+void launch_from_fourier(double* out, const cplx* in, int npolys, const FourierKeys<P>& K,
+                         hipStream_t st);
+template <class P>
+void launch_xai_poly(int32_t* out, hipStream_t st);
+template <class P, bool KB>
+void launch_br_fft(const BrJob* jobs, int njobs, int c, typename P::T mu,
+                   const FourierKeys<P>& K, hipStream_t st);
+void launch_gate_lvl1_fft(bool kb, bool mux, uint32_t* out, const uint32_t* in0,
+                          const uint32_t* in1, const uint32_t* inc, int32_t ca, int32_t cb,
+                          uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
+                          const FourierKeys<lvl1param>& K, hipStream_t st);
+void launch_gate_lvl02_fft(bool kb, bool mux, uint32_t* out, const uint32_t* in0,
+                           const uint32_t* in1, const uint32_t* inc, int32_t ca, int32_t cb,
+                           uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
+                           const FourierKeys<lvl2param>& K, hipStream_t st);
+void launch_ks_batch(int domain_level, bool subset, const KsJob* jobs, int njobs,
+                     const uint32_t* ksk, hipStream_t st);
+void launch_linear(const LinJob* jobs, int njobs, hipStream_t st);
+void launch_linear_one(const LinJob& job, hipStream_t st);
+template <class P>
+void launch_cmux_fft(typename P::T* trlwe, const cplx* trgsw, uint32_t abar,
+                     const FourierKeys<P>& K, hipStream_t st);
+
+}  // namespace cufhe_amd

@@ -1,0 +1,816 @@
+// Runtime behind the C ABI (include/cufhe_amd.h): device selection,
+// transform tables, key upload/conversion, plain per-gate launches and
+// batched plans (optionally captured in a HIP graph).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/cufhe_amd.h"
+#include "../../include/tfhe_params.hpp"
+#include "br_fft.hpp"
+#include "launch.hpp"
+
+namespace cufhe_amd {
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define CUFHE_HIP_TRY(expr)                                                              \
+    do {                                                                                 \
+        const hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess)                                                            \
+            return fail(CUFHE_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class P>
+struct TableSet {
+    cplx* twl = nullptr;
+    cplx* tw64 = nullptr;
+    cplx* cr = nullptr;
+    cplx* e2n = nullptr;
+};
+
+template <class P>
+struct LevelKeys {
+    bool loaded = false;
+    bool kb = false;
+    cplx* bk = nullptr;
+    uint32_t* ksk = nullptr;
+    FourierKeys<P> fk{};
+};
+
+struct State {
+    std::mutex mu;
+    bool initialized = false;
+    int device = 0;
+    TableSet<lvl1param> t1;
+    TableSet<lvl2param> t2;
+    LevelKeys<lvl1param> k1;
+    LevelKeys<lvl2param> k2;
+};
+
+State& state()
+{
+    static State s;
+    return s;
+}
+
+// exp(i*pi*t/N) for t in [0, 2N), computed in long double.
+std::vector<cplx> e2n_table(int N)
+{
+    std::vector<cplx> e(2 * N);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int t = 0; t < 2 * N; t++) {
+        const long double a = pi * static_cast<long double>(t) / static_cast<long double>(N);
+        e[t] = cplx{static_cast<double>(cosl(a)), static_cast<double>(sinl(a))};
+    }
+    // exact values on the axes
+    e[0] = {1.0, 0.0};
+    e[N / 2] = {0.0, 1.0};
+    e[N] = {-1.0, 0.0};
+    e[3 * N / 2] = {0.0, -1.0};
+    return e;
+}
+
+template <class P>
+int build_tables(TableSet<P>& ts)
+{
+    constexpr int N = P::n, H = N / 2, R = H / 64;
+    const std::vector<cplx> E = e2n_table(N);
+    auto at = [&](long long t) { return E[static_cast<size_t>(((t % (2 * N)) + 2 * N) % (2 * N))]; };
+    std::vector<cplx> twl(R * 64), tw64(64), cr(R);
+    // twl[k2*64+l] = exp(i*pi*l/N) * exp(-2*pi*i*l*k2/H) = E[l*(1 - 4*k2)]
+    for (int k2 = 0; k2 < R; k2++)
+        for (int l = 0; l < 64; l++) twl[k2 * 64 + l] = at(static_cast<long long>(l) * (1 - 4 * k2));
+    // tw64[c*8+a] = exp(-2*pi*i*a*c/64) = E[-(2N/64)*a*c]
+    for (int c = 0; c < 8; c++)
+        for (int a = 0; a < 8; a++) tw64[c * 8 + a] = at(-static_cast<long long>(2 * N / 64) * a * c);
+    // cr[r] = exp(i*pi*64*r/N) = E[64 r]
+    for (int r = 0; r < R; r++) cr[r] = at(64LL * r);
+    auto up = [](cplx** dst, const std::vector<cplx>& v) -> hipError_t {
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(dst), v.size() * sizeof(cplx));
+        if (e != hipSuccess) return e;
+        return hipMemcpy(*dst, v.data(), v.size() * sizeof(cplx), hipMemcpyHostToDevice);
+    };
+    CUFHE_HIP_TRY(up(&ts.twl, twl));
+    CUFHE_HIP_TRY(up(&ts.tw64, tw64));
+    CUFHE_HIP_TRY(up(&ts.cr, cr));
+    CUFHE_HIP_TRY(up(&ts.e2n, E));
+    return CUFHE_OK;
+}
+
+template <class P>
+FourierKeys<P> keys_view(const TableSet<P>& ts, const cplx* bk)
+{
+    FourierKeys<P> k;
+    k.bk = bk;
+    k.e2n = ts.e2n;
+    k.fft.twl = ts.twl;
+    k.fft.tw64 = ts.tw64;
+    k.fft.cr = ts.cr;
+    return k;
+}
+
+template <class P>
+void free_tables(TableSet<P>& ts)
+{
+    (void)hipFree(ts.twl);
+    (void)hipFree(ts.tw64);
+    (void)hipFree(ts.cr);
+    (void)hipFree(ts.e2n);
+    ts = TableSet<P>{};
+}
+
+template <class P>
+void free_keys(LevelKeys<P>& k)
+{
+    (void)hipFree(k.bk);
+    (void)hipFree(k.ksk);
+    k = LevelKeys<P>{};
+}
+
+int ensure_init()
+{
+    State& s = state();
+    if (s.initialized) return CUFHE_OK;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return fail(CUFHE_ERR_NO_DEVICE, "no HIP device visible");
+    CUFHE_HIP_TRY(hipSetDevice(s.device));
+    int rc = build_tables(s.t1);
+    if (rc != CUFHE_OK) return rc;
+    rc = build_tables(s.t2);
+    if (rc != CUFHE_OK) return rc;
+    s.initialized = true;
+    return CUFHE_OK;
+}
+
+hipStream_t as_stream(void* st) { return static_cast<hipStream_t>(st); }
+
+int check_launch(const char* what)
+{
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(CUFHE_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return CUFHE_OK;
+}
+
+template <class brP>
+int load_bk(LevelKeys<typename brP::targetP>& lk, const TableSet<typename brP::targetP>& ts,
+            const typename brP::targetP::T* bk, size_t bk_words, bool kb)
+{
+    using P = typename brP::targetP;
+    const size_t ntrgsw = TFHEpp::kBKTRGSWs<brP>(kb);
+    const size_t want = ntrgsw * TFHEpp::kTRGSWWords<P>;
+    if (bk_words != want)
+        return fail(CUFHE_ERR_SIZE, "bootstrapping key has " + std::to_string(bk_words) +
+                                        " words, expected " + std::to_string(want));
+    const size_t npolys = ntrgsw * TFHEpp::BootstrappingTRGSWRows<P> * (P::k + 1);
+    void* tmp = nullptr;
+    CUFHE_HIP_TRY(hipMalloc(&tmp, want * sizeof(typename P::T)));
+    hipError_t e = hipMemcpy(tmp, bk, want * sizeof(typename P::T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(tmp);
+        return fail(CUFHE_ERR_HIP, std::string("bk upload: ") + hipGetErrorString(e));
+    }
+    e = hipMalloc(reinterpret_cast<void**>(&lk.bk), npolys * (P::n / 2) * sizeof(cplx));
+    if (e != hipSuccess) {
+        (void)hipFree(tmp);
+        return fail(CUFHE_ERR_NOMEM, "bootstrapping key spectra allocation failed");
+    }
+    lk.fk = keys_view(ts, lk.bk);
+    launch_to_fourier<P>(lk.bk, tmp, static_cast<int>(npolys), false, lk.fk, nullptr);
+    e = hipDeviceSynchronize();
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(CUFHE_ERR_HIP, std::string("bk conversion: ") + hipGetErrorString(e));
+    lk.kb = kb;
+    return CUFHE_OK;
+}
+
+int load_ksk(uint32_t** dst, const uint32_t* ksk, size_t words, size_t want)
+{
+    if (words != want)
+        return fail(CUFHE_ERR_SIZE, "key-switching key has " + std::to_string(words) +
+                                        " words, expected " + std::to_string(want));
+    CUFHE_HIP_TRY(hipMalloc(reinterpret_cast<void**>(dst), words * sizeof(uint32_t)));
+    CUFHE_HIP_TRY(hipMemcpy(*dst, ksk, words * sizeof(uint32_t), hipMemcpyHostToDevice));
+    return CUFHE_OK;
+}
+
+// Linear combination of a binary gate (reference src/bootstrap_gpu.cu
+// :1335-1881 and :1905-1946): (ca, cb, offset in units of mu).
+struct GateLin {
+    int32_t ca, cb, off_mu;
+};
+bool gate_lin(int op, GateLin& g)
+{
+    switch (op) {
+        case CUFHE_NAND: g = {-1, -1, 1}; return true;
+        case CUFHE_OR: g = {1, 1, 1}; return true;
+        case CUFHE_ORYN: g = {1, -1, 1}; return true;
+        case CUFHE_ORNY: g = {-1, 1, 1}; return true;
+        case CUFHE_AND: g = {1, 1, -1}; return true;
+        case CUFHE_ANDYN: g = {1, -1, -1}; return true;
+        case CUFHE_ANDNY: g = {-1, 1, -1}; return true;
+        case CUFHE_NOR: g = {-1, -1, -1}; return true;
+        case CUFHE_XOR: g = {2, 2, 2}; return true;
+        case CUFHE_XNOR: g = {-2, -2, -2}; return true;
+        default: return false;
+    }
+}
+
+template <class T>
+int alloc_upload(T** dst, const std::vector<T>& v)
+{
+    if (v.empty()) {
+        *dst = nullptr;
+        return CUFHE_OK;
+    }
+    CUFHE_HIP_TRY(hipMalloc(reinterpret_cast<void**>(dst), v.size() * sizeof(T)));
+    CUFHE_HIP_TRY(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return CUFHE_OK;
+}
+
+}  // namespace
+}  // namespace cufhe_amd
+
+using namespace cufhe_amd;
+
+struct cufhe_plan {
+    int njobs_ks = 0, njobs_br = 0, njobs_lin = 0;
+    int c = 2;
+    bool use_graph = false;
+    KsJob* ks = nullptr;
+    BrJob* br = nullptr;
+    LinJob* lin = nullptr;
+    uint32_t* ws = nullptr;  // lvl0 TLWEs + MUX partial results
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+};
+
+namespace {
+
+int plan_launch(cufhe_plan* p, hipStream_t st)
+{
+    State& s = state();
+    if (p->njobs_ks) launch_ks_batch(1, false, p->ks, p->njobs_ks, s.k1.ksk, st);
+    if (p->njobs_br) {
+        if (s.k1.kb)
+            launch_br_fft<lvl1param, true>(p->br, p->njobs_br, p->c,
+                                           static_cast<uint32_t>(lvl1param::μ), s.k1.fk, st);
+        else
+            launch_br_fft<lvl1param, false>(p->br, p->njobs_br, p->c,
+                                            static_cast<uint32_t>(lvl1param::μ), s.k1.fk, st);
+    }
+    if (p->njobs_lin) launch_linear(p->lin, p->njobs_lin, st);
+    return check_launch("batched gate launch");
+}
+
+int pick_c(int c, bool kb, int nbr)
+{
+    if (c == 0) {
+        // autotuned default (bench sweep of {1, 2, 4} on MI355X)
+        c = kb ? 2 : 2;
+        if (nbr < 512) c = 1;
+    }
+    if (c != 1 && c != 2 && c != 4) return -1;
+    if (!kb && c > 2) return -1;
+    return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cufhe_version(void) { return "cufhe_amd 0.1 (gfx950)"; }
+const char* cufhe_last_error(void) { return g_last_error.c_str(); }
+
+int cufhe_set_gpu_num(int gpu_num)
+{
+    if (gpu_num != 1) return fail(CUFHE_ERR_UNSUPPORTED, "one process drives one GPU: gpu_num must be 1");
+    return CUFHE_OK;
+}
+
+int cufhe_initialize(void)
+{
+    std::lock_guard<std::mutex> g(state().mu);
+    return ensure_init();
+}
+
+int cufhe_initialize_lvl01(const uint32_t* bk, size_t bk_words, int key_bundle,
+                           const uint32_t* ksk, size_t ksk_words)
+{
+    if (!bk || !ksk) return fail(CUFHE_ERR_ARG, "null key pointer");
+    State& s = state();
+    std::lock_guard<std::mutex> g(s.mu);
+    int rc = ensure_init();
+    if (rc != CUFHE_OK) return rc;
+    free_keys(s.k1);
+    rc = load_bk<TFHEpp::lvl01param>(s.k1, s.t1, bk, bk_words, key_bundle != 0);
+    if (rc == CUFHE_OK) rc = load_ksk(&s.k1.ksk, ksk, ksk_words, TFHEpp::kKSKWords<TFHEpp::lvl10param>);
+    if (rc != CUFHE_OK) {
+        free_keys(s.k1);
+        return rc;
+    }
+    s.k1.loaded = true;
+    return CUFHE_OK;
+}
+
+int cufhe_initialize_lvl02(const uint64_t* bk, size_t bk_words, int key_bundle,
+                           const uint32_t* ksk, size_t ksk_words)
+{
+    if (!bk || !ksk) return fail(CUFHE_ERR_ARG, "null key pointer");
+    State& s = state();
+    std::lock_guard<std::mutex> g(s.mu);
+    int rc = ensure_init();
+    if (rc != CUFHE_OK) return rc;
+    free_keys(s.k2);
+    rc = load_bk<TFHEpp::lvl02param>(s.k2, s.t2, bk, bk_words, key_bundle != 0);
+    if (rc == CUFHE_OK) rc = load_ksk(&s.k2.ksk, ksk, ksk_words, TFHEpp::kKSKWords<TFHEpp::lvl20param>);
+    if (rc != CUFHE_OK) {
+        free_keys(s.k2);
+        return rc;
+    }
+    s.k2.loaded = true;
+    return CUFHE_OK;
+}
+
+int cufhe_cleanup(void)
+{
+    State& s = state();
+    std::lock_guard<std::mutex> g(s.mu);
+    if (!s.initialized) return CUFHE_OK;
+    (void)hipDeviceSynchronize();
+    free_keys(s.k1);
+    free_keys(s.k2);
+    free_tables(s.t1);
+    free_tables(s.t2);
+    s.initialized = false;
+    return CUFHE_OK;
+}
+
+int cufhe_cleanup_lvl02(void)
+{
+    State& s = state();
+    std::lock_guard<std::mutex> g(s.mu);
+    (void)hipDeviceSynchronize();
+    free_keys(s.k2);
+    return CUFHE_OK;
+}
+
+int cufhe_synchronize(void)
+{
+    CUFHE_HIP_TRY(hipDeviceSynchronize());
+    return CUFHE_OK;
+}
+
+int cufhe_stream_create(void** stream)
+{
+    if (!stream) return fail(CUFHE_ERR_ARG, "null stream handle");
+    hipStream_t st;
+    CUFHE_HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    *stream = st;
+    return CUFHE_OK;
+}
+
+int cufhe_stream_destroy(void* stream)
+{
+    if (!stream) return CUFHE_OK;
+    CUFHE_HIP_TRY(hipStreamDestroy(as_stream(stream)));
+    return CUFHE_OK;
+}
+
+int cufhe_stream_query(void* stream)
+{
+    const hipError_t e = hipStreamQuery(as_stream(stream));
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return fail(CUFHE_ERR_HIP, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+}
+
+int cufhe_stream_synchronize(void* stream)
+{
+    CUFHE_HIP_TRY(hipStreamSynchronize(as_stream(stream)));
+    return CUFHE_OK;
+}
+
+int cufhe_device_alloc(void** ptr, size_t bytes)
+{
+    if (!ptr) return fail(CUFHE_ERR_ARG, "null pointer");
+    if (hipMalloc(ptr, bytes) != hipSuccess) return fail(CUFHE_ERR_NOMEM, "hipMalloc failed");
+    return CUFHE_OK;
+}
+
+int cufhe_device_free(void* ptr)
+{
+    CUFHE_HIP_TRY(hipFree(ptr));
+    return CUFHE_OK;
+}
+
+int cufhe_host_alloc(void** ptr, size_t bytes)
+{
+    if (!ptr) return fail(CUFHE_ERR_ARG, "null pointer");
+    if (hipHostMalloc(ptr, bytes, hipHostMallocDefault) != hipSuccess)
+        return fail(CUFHE_ERR_NOMEM, "hipHostMalloc failed");
+    return CUFHE_OK;
+}
+
+int cufhe_host_free(void* ptr)
+{
+    CUFHE_HIP_TRY(hipHostFree(ptr));
+    return CUFHE_OK;
+}
+
+int cufhe_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream)
+{
+    CUFHE_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+    return CUFHE_OK;
+}
+
+int cufhe_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream)
+{
+    CUFHE_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, as_stream(stream)));
+    return CUFHE_OK;
+}
+
+int cufhe_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream)
+{
+    CUFHE_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
+    return CUFHE_OK;
+}
+
+int cufhe_memset_d(void* dst, int value, size_t bytes, void* stream)
+{
+    CUFHE_HIP_TRY(hipMemsetAsync(dst, value, bytes, as_stream(stream)));
+    return CUFHE_OK;
+}
+
+int cufhe_gate_lvl1(int op, uint32_t* out, const uint32_t* in0, const uint32_t* in1,
+                    const uint32_t* inc, void* stream)
+{
+    State& s = state();
+    if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded (cufhe_initialize_lvl01)");
+    if (!out || !in0) return fail(CUFHE_ERR_ARG, "null ciphertext pointer");
+    const hipStream_t st = as_stream(stream);
+    constexpr uint32_t mu = static_cast<uint32_t>(lvl1param::μ);
+    if (op == CUFHE_NOT || op == CUFHE_COPY) {
+        launch_linear_one(LinJob{in0, nullptr, out, op == CUFHE_NOT ? -1 : 1, 0, 0u,
+                                 static_cast<int32_t>(lvl1param::n + 1)},
+                          st);
+        return check_launch("not/copy");
+    }
+    if (op == CUFHE_MUX || op == CUFHE_NMUX) {
+        if (!in1 || !inc) return fail(CUFHE_ERR_ARG, "null MUX operand");
+        launch_gate_lvl1_fft(s.k1.kb, true, out, in0, in1, inc, 0, 0, 0u,
+                             op == CUFHE_MUX ? 1 : -1, s.k1.ksk, s.k1.fk, st);
+        return check_launch("mux gate");
+    }
+    GateLin gl;
+    if (!gate_lin(op, gl)) return fail(CUFHE_ERR_ARG, "unknown gate op " + std::to_string(op));
+    if (!in1) return fail(CUFHE_ERR_ARG, "null second operand");
+    launch_gate_lvl1_fft(s.k1.kb, false, out, in0, in1, nullptr, gl.ca, gl.cb,
+                         static_cast<uint32_t>(gl.off_mu) * mu, 0, s.k1.ksk, s.k1.fk, st);
+    return check_launch("gate");
+}
+
+int cufhe_gate_lvl02(int op, uint32_t* out, const uint32_t* in0, const uint32_t* in1,
+                     const uint32_t* inc, void* stream)
+{
+    State& s = state();
+    if (!s.k2.loaded) return fail(CUFHE_ERR_STATE, "lvl02 keys not loaded (cufhe_initialize_lvl02)");
+    if (!out || !in0) return fail(CUFHE_ERR_ARG, "null ciphertext pointer");
+    const hipStream_t st = as_stream(stream);
+    constexpr uint32_t mu0 = static_cast<uint32_t>(lvl0param::μ);
+    if (op == CUFHE_NOT || op == CUFHE_COPY) {
+        launch_linear_one(LinJob{in0, nullptr, out, op == CUFHE_NOT ? -1 : 1, 0, 0u,
+                                 static_cast<int32_t>(lvl0param::n + 1)},
+                          st);
+        return check_launch("not/copy lvl0");
+    }
+    if (op == CUFHE_MUX || op == CUFHE_NMUX) {
+        if (!in1 || !inc) return fail(CUFHE_ERR_ARG, "null MUX operand");
+        launch_gate_lvl02_fft(s.k2.kb, true, out, in0, in1, inc, 0, 0, 0u,
+                              op == CUFHE_MUX ? 1 : -1, s.k2.ksk, s.k2.fk, st);
+        return check_launch("mux gate lvl02");
+    }
+    GateLin gl;
+    if (!gate_lin(op, gl)) return fail(CUFHE_ERR_ARG, "unknown gate op " + std::to_string(op));
+    if (!in1) return fail(CUFHE_ERR_ARG, "null second operand");
+    launch_gate_lvl02_fft(s.k2.kb, false, out, in0, in1, nullptr, gl.ca, gl.cb,
+                          static_cast<uint32_t>(gl.off_mu) * mu0, 0, s.k2.ksk, s.k2.fk, st);
+    return check_launch("gate lvl02");
+}
+
+int cufhe_plan_create_lvl1(const cufhe_gate_desc* gates, int n, uint32_t* d_pool,
+                           int jobs_per_group, int use_graph, cufhe_plan** out)
+{
+    State& s = state();
+    if (!out) return fail(CUFHE_ERR_ARG, "null plan handle");
+    *out = nullptr;
+    if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded (cufhe_initialize_lvl01)");
+    if (n < 0 || (n > 0 && (!gates || !d_pool))) return fail(CUFHE_ERR_ARG, "bad gate list");
+    constexpr uint32_t W1 = lvl1param::n + 1;
+    constexpr uint32_t W0 = 640;  // lvl0 TLWE stride in the workspace
+    constexpr uint32_t mu = static_cast<uint32_t>(lvl1param::μ);
+    int nks = 0, nmux = 0;
+    for (int i = 0; i < n; i++) {
+        const int op = gates[i].op;
+        if (op < 0 || op >= CUFHE_GATE_COUNT) return fail(CUFHE_ERR_ARG, "unknown gate op");
+        if (op == CUFHE_MUX || op == CUFHE_NMUX) {
+            nks += 2;
+            nmux++;
+        }
+        else if (op != CUFHE_NOT && op != CUFHE_COPY) {
+            nks += 1;
+        }
+    }
+    const int c = pick_c(jobs_per_group, s.k1.kb, nks);
+    if (c < 0) return fail(CUFHE_ERR_UNSUPPORTED, "jobs_per_group must be 0, 1, 2 or 4 (KB) / 1, 2");
+    cufhe_plan* p = new cufhe_plan();
+    p->c = c;
+    p->use_graph = use_graph != 0;
+    const size_t ws_words = static_cast<size_t>(nks) * W0 + static_cast<size_t>(nmux) * 2 * W1;
+    if (ws_words) {
+        if (hipMalloc(reinterpret_cast<void**>(&p->ws), ws_words * sizeof(uint32_t)) != hipSuccess) {
+            delete p;
+            return fail(CUFHE_ERR_NOMEM, "plan workspace allocation failed");
+        }
+    }
+    std::vector<KsJob> ks;
+    std::vector<BrJob> br;
+    std::vector<LinJob> lin;
+    ks.reserve(nks);
+    br.reserve(nks);
+    uint32_t* lvl0 = p->ws;
+    uint32_t* res = p->ws + static_cast<size_t>(nks) * W0;
+    int mux_i = 0;
+    auto wire = [&](uint32_t w) { return d_pool + static_cast<size_t>(w) * W1; };
+    for (int i = 0; i < n; i++) {
+        const cufhe_gate_desc& gd = gates[i];
+        if (gd.op == CUFHE_NOT || gd.op == CUFHE_COPY) {
+            lin.push_back(LinJob{wire(gd.in0), nullptr, wire(gd.out), gd.op == CUFHE_NOT ? -1 : 1, 0,
+                                 0u, static_cast<int32_t>(W1)});
+            continue;
+        }
+        if (gd.op == CUFHE_MUX || gd.op == CUFHE_NMUX) {
+            uint32_t* r1 = res + static_cast<size_t>(2 * mux_i) * W1;
+            uint32_t* r0 = r1 + W1;
+            mux_i++;
+            const int sgn = gd.op == CUFHE_MUX ? 1 : -1;
+            for (int half = 0; half < 2; half++) {
+                uint32_t* l0 = lvl0 + static_cast<size_t>(ks.size()) * W0;
+                ks.push_back(KsJob{wire(gd.inc), wire(half == 0 ? gd.in1 : gd.in0), l0,
+                                   half == 0 ? 1 : -1, 1, static_cast<uint64_t>(0u - mu)});
+                br.push_back(BrJob{l0, nullptr, half == 0 ? r1 : r0, 1, 0, 0u, 0u});
+            }
+            lin.push_back(LinJob{r1, r0, wire(gd.out), sgn, sgn, static_cast<uint32_t>(sgn) * mu,
+                                 static_cast<int32_t>(W1)});
+            continue;
+        }
+        GateLin gl;
+        gate_lin(gd.op, gl);
+        uint32_t* l0 = lvl0 + static_cast<size_t>(ks.size()) * W0;
+        ks.push_back(KsJob{wire(gd.in0), wire(gd.in1), l0, gl.ca, gl.cb,
+                           static_cast<uint64_t>(static_cast<uint32_t>(gl.off_mu) * mu)});
+        br.push_back(BrJob{l0, nullptr, wire(gd.out), 1, 0, 0u, 0u});
+    }
+    int rc = alloc_upload(&p->ks, ks);
+    if (rc == CUFHE_OK) rc = alloc_upload(&p->br, br);
+    if (rc == CUFHE_OK) rc = alloc_upload(&p->lin, lin);
+    if (rc != CUFHE_OK) {
+        cufhe_plan_destroy(p);
+        return rc;
+    }
+    p->njobs_ks = static_cast<int>(ks.size());
+    p->njobs_br = static_cast<int>(br.size());
+    p->njobs_lin = static_cast<int>(lin.size());
+    *out = p;
+    return CUFHE_OK;
+}
+
+int cufhe_plan_run(cufhe_plan* p, void* stream)
+{
+    if (!p) return fail(CUFHE_ERR_ARG, "null plan");
+    State& s = state();
+    if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded");
+    const hipStream_t st = as_stream(stream);
+    if (!p->use_graph) return plan_launch(p, st);
+    if (!p->exec) {
+        // Capture on a private stream; the graph is replayed on `stream`.
+        hipStream_t cs;
+        CUFHE_HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        CUFHE_HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        const int rc = plan_launch(p, cs);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(cs, &g);
+        (void)hipStreamDestroy(cs);
+        if (rc != CUFHE_OK) return rc;
+        if (e != hipSuccess) return fail(CUFHE_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        p->graph = g;
+        CUFHE_HIP_TRY(hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
+    }
+    CUFHE_HIP_TRY(hipGraphLaunch(p->exec, st));
+    return CUFHE_OK;
+}
+
+int cufhe_plan_destroy(cufhe_plan* p)
+{
+    if (!p) return CUFHE_OK;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    (void)hipFree(p->ks);
+    (void)hipFree(p->br);
+    (void)hipFree(p->lin);
+    (void)hipFree(p->ws);
+    delete p;
+    return CUFHE_OK;
+}
+
+int cufhe_plan_bootstraps(const cufhe_plan* p) { return p ? p->njobs_br : 0; }
+
+int cufhe_plan_jobs_per_group(const cufhe_plan* p) { return p ? p->c : 0; }
+
+int cufhe_plan_run_timed(cufhe_plan* p, void* stream, float* kernel_ms)
+{
+    if (!p || !kernel_ms) return fail(CUFHE_ERR_ARG, "null argument");
+    State& s = state();
+    if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded");
+    const hipStream_t st = as_stream(stream);
+    hipEvent_t ev[4];
+    for (auto& e : ev) CUFHE_HIP_TRY(hipEventCreate(&e));
+    CUFHE_HIP_TRY(hipEventRecord(ev[0], st));
+    if (p->njobs_ks) launch_ks_batch(1, false, p->ks, p->njobs_ks, s.k1.ksk, st);
+    CUFHE_HIP_TRY(hipEventRecord(ev[1], st));
+    if (p->njobs_br) {
+        if (s.k1.kb)
+            launch_br_fft<lvl1param, true>(p->br, p->njobs_br, p->c,
+                                           static_cast<uint32_t>(lvl1param::μ), s.k1.fk, st);
+        else
+            launch_br_fft<lvl1param, false>(p->br, p->njobs_br, p->c,
+                                            static_cast<uint32_t>(lvl1param::μ), s.k1.fk, st);
+    }
+    CUFHE_HIP_TRY(hipEventRecord(ev[2], st));
+    if (p->njobs_lin) launch_linear(p->lin, p->njobs_lin, st);
+    CUFHE_HIP_TRY(hipEventRecord(ev[3], st));
+    int rc = check_launch("timed plan");
+    if (rc == CUFHE_OK) {
+        const hipError_t e = hipEventSynchronize(ev[3]);
+        if (e != hipSuccess) rc = fail(CUFHE_ERR_HIP, std::string("timed plan: ") + hipGetErrorString(e));
+    }
+    if (rc == CUFHE_OK)
+        for (int i = 0; i < 3; i++) (void)hipEventElapsedTime(&kernel_ms[i], ev[i], ev[i + 1]);
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+int cufhe_gate_batch_lvl1(const cufhe_gate_desc* gates, int n, uint32_t* d_pool,
+                          int jobs_per_group, void* stream)
+{
+    cufhe_plan* p = nullptr;
+    int rc = cufhe_plan_create_lvl1(gates, n, d_pool, jobs_per_group, 0, &p);
+    if (rc != CUFHE_OK) return rc;
+    rc = cufhe_plan_run(p, stream);
+    if (rc == CUFHE_OK) {
+        const hipError_t e = hipStreamSynchronize(as_stream(stream));
+        if (e != hipSuccess) rc = fail(CUFHE_ERR_HIP, std::string("batch: ") + hipGetErrorString(e));
+    }
+    cufhe_plan_destroy(p);
+    return rc;
+}
+
+int cufhe_identity_key_switch_lvl10(uint32_t* d_out, const uint32_t* d_in, int n, void* stream)
+{
+    State& s = state();
+    if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded");
+    if (n <= 0) return n == 0 ? CUFHE_OK : fail(CUFHE_ERR_ARG, "negative count");
+    if (!d_out || !d_in) return fail(CUFHE_ERR_ARG, "null pointer");
+    const hipStream_t st = as_stream(stream);
+    std::vector<KsJob> jobs(n);
+    for (int i = 0; i < n; i++)
+        jobs[i] = KsJob{d_in + static_cast<size_t>(i) * (lvl1param::n + 1), nullptr,
+                        d_out + static_cast<size_t>(i) * (lvl0param::n + 1), 1, 0, 0};
+    KsJob* d = nullptr;
+    CUFHE_HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&d), n * sizeof(KsJob), st));
+    CUFHE_HIP_TRY(hipMemcpyAsync(d, jobs.data(), n * sizeof(KsJob), hipMemcpyHostToDevice, st));
+    launch_ks_batch(1, false, d, n, s.k1.ksk, st);
+    CUFHE_HIP_TRY(hipFreeAsync(d, st));
+    CUFHE_HIP_TRY(hipStreamSynchronize(st));
+    return check_launch("identity key switch");
+}
+
+int cufhe_bootstrap_lvl01(uint32_t* d_out, const uint32_t* d_in, uint32_t mu, int n,
+                          int jobs_per_group, void* stream)
+{
+    State& s = state();
+    if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded");
+    if (n <= 0) return n == 0 ? CUFHE_OK : fail(CUFHE_ERR_ARG, "negative count");
+    if (!d_out || !d_in) return fail(CUFHE_ERR_ARG, "null pointer");
+    const int c = pick_c(jobs_per_group, s.k1.kb, n);
+    if (c < 0) return fail(CUFHE_ERR_UNSUPPORTED, "unsupported jobs_per_group");
+    const hipStream_t st = as_stream(stream);
+    std::vector<BrJob> jobs(n);
+    for (int i = 0; i < n; i++)
+        jobs[i] = BrJob{d_in + static_cast<size_t>(i) * (lvl0param::n + 1), nullptr,
+                        d_out + static_cast<size_t>(i) * (lvl1param::n + 1), 1, 0, 0u, 0u};
+    BrJob* d = nullptr;
+    CUFHE_HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&d), n * sizeof(BrJob), st));
+    CUFHE_HIP_TRY(hipMemcpyAsync(d, jobs.data(), n * sizeof(BrJob), hipMemcpyHostToDevice, st));
+    if (s.k1.kb)
+        launch_br_fft<lvl1param, true>(d, n, c, mu, s.k1.fk, st);
+    else
+        launch_br_fft<lvl1param, false>(d, n, c, mu, s.k1.fk, st);
+    CUFHE_HIP_TRY(hipFreeAsync(d, st));
+    CUFHE_HIP_TRY(hipStreamSynchronize(st));
+    return check_launch("bootstrap");
+}
+
+int cufhe_ffnt_forward(int N, const void* d_poly, int integer, void* d_spectra, int count,
+                       void* stream)
+{
+    State& s = state();
+    {
+        std::lock_guard<std::mutex> g(s.mu);
+        const int rc = ensure_init();
+        if (rc != CUFHE_OK) return rc;
+    }
+    if (count < 0 || !d_poly || !d_spectra) return fail(CUFHE_ERR_ARG, "bad arguments");
+    if (count == 0) return CUFHE_OK;
+    if (N == 1024)
+        launch_to_fourier<lvl1param>(static_cast<cplx*>(d_spectra), d_poly, count, integer != 0,
+                                     keys_view(s.t1, nullptr), as_stream(stream));
+    else if (N == 2048)
+        launch_to_fourier<lvl2param>(static_cast<cplx*>(d_spectra), d_poly, count, integer != 0,
+                                     keys_view(s.t2, nullptr), as_stream(stream));
+    else
+        return fail(CUFHE_ERR_UNSUPPORTED, "N must be 1024 or 2048");
+    return check_launch("ffnt forward");
+}
+
+int cufhe_ffnt_inverse(int N, const void* d_spectra, double* d_poly, int count, void* stream)
+{
+    State& s = state();
+    {
+        std::lock_guard<std::mutex> g(s.mu);
+        const int rc = ensure_init();
+        if (rc != CUFHE_OK) return rc;
+    }
+    if (count < 0 || !d_poly || !d_spectra) return fail(CUFHE_ERR_ARG, "bad arguments");
+    if (count == 0) return CUFHE_OK;
+    if (N == 1024)
+        launch_from_fourier<lvl1param>(d_poly, static_cast<const cplx*>(d_spectra), count,
+                                       keys_view(s.t1, nullptr), as_stream(stream));
+    else if (N == 2048)
+        launch_from_fourier<lvl2param>(d_poly, static_cast<const cplx*>(d_spectra), count,
+                                       keys_view(s.t2, nullptr), as_stream(stream));
+    else
+        return fail(CUFHE_ERR_UNSUPPORTED, "N must be 1024 or 2048");
+    return check_launch("ffnt inverse");
+}
+
+int cufhe_trgsw_to_fourier(int N, const void* d_trgsw, void* d_spectra, int count, void* stream)
+{
+    if (N == 1024)
+        return cufhe_ffnt_forward(N, d_trgsw, 0, d_spectra,
+                                  count * TFHEpp::BootstrappingTRGSWRows<lvl1param> * 2, stream);
+    if (N == 2048)
+        return cufhe_ffnt_forward(N, d_trgsw, 0, d_spectra,
+                                  count * TFHEpp::BootstrappingTRGSWRows<lvl2param> * 2, stream);
+    return fail(CUFHE_ERR_UNSUPPORTED, "N must be 1024 or 2048");
+}
+
+int cufhe_cmux_accumulate(int N, void* d_trlwe, const void* d_trgsw_spectra, uint32_t abar,
+                          void* stream)
+{
+    State& s = state();
+    {
+        std::lock_guard<std::mutex> g(s.mu);
+        const int rc = ensure_init();
+        if (rc != CUFHE_OK) return rc;
+    }
+    if (!d_trlwe || !d_trgsw_spectra) return fail(CUFHE_ERR_ARG, "null pointer");
+    if (abar >= 2u * static_cast<uint32_t>(N)) return fail(CUFHE_ERR_ARG, "abar out of [0, 2N)");
+    if (N == 1024)
+        launch_cmux_fft<lvl1param>(static_cast<uint32_t*>(d_trlwe),
+                                   static_cast<const cplx*>(d_trgsw_spectra), abar,
+                                   keys_view(s.t1, nullptr), as_stream(stream));
+    else if (N == 2048)
+        launch_cmux_fft<lvl2param>(static_cast<uint64_t*>(d_trlwe),
+                                   static_cast<const cplx*>(d_trgsw_spectra), abar,
+                                   keys_view(s.t2, nullptr), as_stream(stream));
+    else
+        return fail(CUFHE_ERR_UNSUPPORTED, "N must be 1024 or 2048");
+    return check_launch("cmux");
+}
+
+}  // extern "C"

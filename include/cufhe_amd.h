@@ -1,0 +1,182 @@
+/*
+ * C ABI of the MI355X gate-bootstrapping engine: the drop-in boundary for
+ * cuFHEpp's gate path. Plain pointers, sizes and opaque stream handles only.
+ *
+ * Ciphertext layouts (TFHEpp, see include/tfhe_params.hpp):
+ *   lvl1 TLWE: 1025 x uint32 (a[0..1023], b), lvl0 TLWE: 637 x uint32,
+ *   lvl2 TLWE: 2049 x uint64.
+ * Device pointers come from cufhe_device_alloc (or any hipMalloc). A stream
+ * handle is a hipStream_t passed as void* (NULL = default stream).
+ * Every entry point returns CUFHE_OK (0) or a negative CUFHE_ERR_* code;
+ * cufhe_last_error() describes the last failure of the calling thread. The
+ * reference prints and calls exit(-1) instead (include/error_gpu.cuh:30-66);
+ * the C++ layer (include/cufhe_gpu.hpp) restores that behaviour.
+ *
+ * Reference interfaces each entry point replaces are cited per function
+ * (paths relative to the cuFHEpp source tree).
+ */
+#ifndef CUFHE_AMD_H
+#define CUFHE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUFHE_OK 0
+#define CUFHE_ERR_ARG (-1)         /* bad argument (null pointer, bad op) */
+#define CUFHE_ERR_SIZE (-2)        /* buffer size does not match the layout */
+#define CUFHE_ERR_STATE (-3)       /* not initialised / keys not loaded */
+#define CUFHE_ERR_HIP (-4)         /* HIP runtime error */
+#define CUFHE_ERR_UNSUPPORTED (-5) /* configuration not provided */
+#define CUFHE_ERR_NOMEM (-6)       /* device allocation failed */
+#define CUFHE_ERR_NO_DEVICE (-7)   /* no GPU visible */
+
+/* Gate codes (reference include/cufhe_gpu.cuh:249-304 gate set). */
+typedef enum {
+    CUFHE_NAND = 0,
+    CUFHE_OR = 1,
+    CUFHE_ORYN = 2,
+    CUFHE_ORNY = 3,
+    CUFHE_AND = 4,
+    CUFHE_ANDYN = 5,
+    CUFHE_ANDNY = 6,
+    CUFHE_NOR = 7,
+    CUFHE_XOR = 8,
+    CUFHE_XNOR = 9,
+    CUFHE_MUX = 10,  /* out = inc ? in1 : in0 */
+    CUFHE_NMUX = 11, /* out = !(inc ? in1 : in0) */
+    CUFHE_NOT = 12,
+    CUFHE_COPY = 13,
+    CUFHE_GATE_COUNT = 14
+} cufhe_gate_t;
+
+/* One gate of a batch: wire indices into a device pool of TLWEs. */
+typedef struct {
+    int32_t op;
+    uint32_t out;
+    uint32_t in0;
+    uint32_t in1;
+    uint32_t inc; /* MUX/NMUX selector; ignored otherwise */
+} cufhe_gate_desc;
+
+typedef struct cufhe_plan cufhe_plan;
+
+/* ---- version / errors --------------------------------------------------- */
+const char* cufhe_version(void);
+const char* cufhe_last_error(void);
+
+/* ---- setup (reference include/cufhe_gpu.cuh:52-70,
+ *      src/cufhe_gates_gpu.cu:38-103) ---------------------------------- */
+
+/* SetGPUNum: only 1 is supported (one process per GPU). */
+int cufhe_set_gpu_num(int gpu_num);
+/* Initialize(): select the device and build the transform tables. */
+int cufhe_initialize(void);
+/* Initialize(ek) for the lvl1 gate API: bk is TFHEpp's polynomial-domain
+ * BootstrappingKey<lvl01param> (tfhe_bk_words(1, key_bundle) words), ksk is
+ * KeySwitchingKey<lvl10param> (tfhe_ksk_words(1) words). The key is
+ * converted to the Fourier domain on the GPU. */
+int cufhe_initialize_lvl01(const uint32_t* bk, size_t bk_words, int key_bundle,
+                           const uint32_t* ksk, size_t ksk_words);
+/* Initialize_lvl02(ek, sk): BootstrappingKey<lvl02param> (uint64 words) and
+ * KeySwitchingKey<lvl20param>. */
+int cufhe_initialize_lvl02(const uint64_t* bk, size_t bk_words, int key_bundle,
+                           const uint32_t* ksk, size_t ksk_words);
+/* CleanUp() / CleanUp_lvl02(). */
+int cufhe_cleanup(void);
+int cufhe_cleanup_lvl02(void);
+/* Synchronize(): wait for all work on the device. */
+int cufhe_synchronize(void);
+
+/* ---- streams (reference include/cufhe_gpu.cuh:162-201 Stream,
+ *      src/cufhe_gates_gpu.cu:104-114 StreamQuery) ------------------------ */
+int cufhe_stream_create(void** stream);
+int cufhe_stream_destroy(void* stream);
+/* 1 when all work on the stream is complete, 0 when work is pending. */
+int cufhe_stream_query(void* stream);
+int cufhe_stream_synchronize(void* stream);
+
+/* ---- memory (reference Ctxt ctor/dtor include/cufhe_gpu.cuh:78-121 and
+ *      CtxtCopyH2D/D2H :203-217) --------------------------------------- */
+int cufhe_device_alloc(void** ptr, size_t bytes);
+int cufhe_device_free(void* ptr);
+int cufhe_host_alloc(void** ptr, size_t bytes); /* pinned */
+int cufhe_host_free(void* ptr);
+int cufhe_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int cufhe_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int cufhe_memset_d(void* dst, int value, size_t bytes, void* stream);
+int cufhe_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+
+/* ---- plain per-gate form (reference src/cufhe_gates_gpu.cu:514-537 g*
+ *      gates; src/bootstrap_gpu.cu:2416-3884 XxxBootstrap launchers) ----- */
+
+/* lvl1 API, IKS-BR order: one gate per launch on `stream`. All pointers are
+ * device lvl1 TLWEs; in1 is unused for NOT/COPY, inc only for MUX/NMUX.
+ * out may alias any input. */
+int cufhe_gate_lvl1(int op, uint32_t* out, const uint32_t* in0, const uint32_t* in1,
+                    const uint32_t* inc, void* stream);
+/* lvl02 API, BR-IKS order: lvl0 TLWEs in and out, bootstrapped at lvl2
+ * (reference Nand_lvl02 ... NMux_lvl02, src/cufhe_gates_gpu.cu:826-977). */
+int cufhe_gate_lvl02(int op, uint32_t* out, const uint32_t* in0, const uint32_t* in1,
+                     const uint32_t* inc, void* stream);
+
+/* ---- batched form (no reference counterpart: cuFHEpp runs one gate per
+ *      stream; this is the batch-amortised path behind the same gates) --- */
+
+/* Plan a batch of n gates over a device pool of lvl1 TLWEs (wire w at
+ * d_pool + 1025 w). Outputs of a batch must not be read by the same batch.
+ * jobs_per_group: bootstraps sharing one key pass per workgroup (1, 2, 4;
+ * 0 = autotuned choice). use_graph != 0 captures the launches in a HIP
+ * graph on the first run and replays it. */
+int cufhe_plan_create_lvl1(const cufhe_gate_desc* gates, int n, uint32_t* d_pool,
+                           int jobs_per_group, int use_graph, cufhe_plan** plan);
+int cufhe_plan_run(cufhe_plan* plan, void* stream);
+int cufhe_plan_destroy(cufhe_plan* plan);
+/* Bootstraps (blind rotations) a plan performs per run. */
+int cufhe_plan_bootstraps(const cufhe_plan* plan);
+/* Jobs per workgroup the plan launches with (after autotuning). */
+int cufhe_plan_jobs_per_group(const cufhe_plan* plan);
+/* Run the plan eagerly with HIP events around each kernel on `stream`,
+ * wait for it, and return the kernel durations in ms:
+ * kernel_ms[0] = key switching, [1] = blind rotation, [2] = combine. */
+int cufhe_plan_run_timed(cufhe_plan* plan, void* stream, float* kernel_ms);
+/* Convenience: plan + run + destroy (allocates per call). */
+int cufhe_gate_batch_lvl1(const cufhe_gate_desc* gates, int n, uint32_t* d_pool,
+                          int jobs_per_group, void* stream);
+
+/* ---- components ----------------------------------------------------------
+ * IdentityKeySwitch lvl1 -> lvl0 of n ciphertexts (reference
+ * include/keyswitch_gpu.cuh:128-187 KeySwitchFromTLWE / SEIandKS
+ * src/keyswitch_gpu.cu:61-68). in: n x 1025, out: n x 637 words. */
+int cufhe_identity_key_switch_lvl10(uint32_t* d_out, const uint32_t* d_in, int n, void* stream);
+/* Bootstrap: blind rotation of n lvl0 TLWEs with test vector +-mu and
+ * sample extraction to lvl1 (reference src/bootstrap_gpu.cu:2346-2355).
+ * in: n x 637, out: n x 1025 words. */
+int cufhe_bootstrap_lvl01(uint32_t* d_out, const uint32_t* d_in, uint32_t mu, int n,
+                          int jobs_per_group, void* stream);
+
+/* FFNT of count real polynomials of degree N (1024 or 2048). integer != 0:
+ * int32 coefficients used as is; else torus words (uint32 for N=1024,
+ * uint64 for N=2048) scaled by 2^-w. Spectra: count x N/2 x (double re, im)
+ * in the engine's bin order (reference src/bootstrap_gpu.cu:59-112). */
+int cufhe_ffnt_forward(int N, const void* d_poly, int integer, void* d_spectra, int count,
+                       void* stream);
+/* Inverse FFNT incl. 1/(N/2): spectra -> count x N doubles. */
+int cufhe_ffnt_inverse(int N, const void* d_spectra, double* d_poly, int count, void* stream);
+/* TRGSW (polynomial domain, TFHEpp layout) -> spectra, count TRGSWs. */
+int cufhe_trgsw_to_fourier(int N, const void* d_trgsw, void* d_spectra, int count,
+                           void* stream);
+/* One CMUX accumulate step acc += ExtProd((X^abar - 1) acc, C) on a device
+ * TRLWE (k+1 polys) with C in the Fourier domain (reference
+ * include/gatebootstrapping_gpu.cuh:69-221 Accumulate). */
+int cufhe_cmux_accumulate(int N, void* d_trlwe, const void* d_trgsw_spectra, uint32_t abar,
+                          void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUFHE_AMD_H */

@@ -1,0 +1,76 @@
+/*
+ * Host-side TFHE layer: seeded key generation, encryption and decryption in
+ * the memory layouts the gate-bootstrapping path consumes.
+ *
+ * The reference takes all of this from TFHEpp (an un-vendored submodule,
+ * reference .gitmodules:1-3): SecretKey, EvalKey::emplacebk/emplaceiksk,
+ * tlweSymEncrypt/Decrypt (used at reference test/test_util.h:25-27, 77-78 and
+ * src/cufhe_gates_gpu.cu:42-61). These entry points are this project's own
+ * replacement, so keys and ciphertexts are reproducible from a seed.
+ *
+ * Levels: 0 -> lvl0 (n=636, uint32), 1 -> lvl1 (N=1024, uint32),
+ *         2 -> lvl2 (N=2048, uint64).
+ * All functions return 0 on success and a negative CUFHE_ERR_* code on bad
+ * arguments. None of them touches a GPU.
+ */
+#ifndef TFHE_HOST_H
+#define TFHE_HOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Secret keys. lvl0: 636 words, lvl1: 1024 words, lvl2: 2048 words (u64).
+ * block_binary != 0 draws the lvl0 key in blocks of 4 with at most one bit
+ * set per block and makes the lvl1 key start with the lvl0 key. */
+int tfhe_secret_key_gen(uint64_t seed, int block_binary, uint32_t* key_lvl0,
+                        uint32_t* key_lvl1, uint64_t* key_lvl2);
+
+/* Bootstrapping key in the polynomial domain (TFHEpp BootstrappingKey
+ * layout): target_level 1 -> uint32 words, 2 -> uint64 words.
+ * key_bundle != 0 writes n/2 groups of 3 TRGSWs:
+ *   [3i+0] = TRGSW(s_{2i} s_{2i+1}), [3i+1] = TRGSW(s_{2i}(1-s_{2i+1})),
+ *   [3i+2] = TRGSW((1-s_{2i}) s_{2i+1}).
+ * Otherwise n TRGSWs, [i] = TRGSW(s_i). Words written are returned through
+ * *words_out when it is non-NULL. */
+int tfhe_bk_gen(int target_level, int key_bundle, uint64_t seed,
+                const uint32_t* key_lvl0, const void* key_target, void* bk_out,
+                size_t bk_capacity_words, size_t* words_out);
+
+/* Key-switching key, layout [j][digit][|v|-1][0..n0] of uint32.
+ * domain_level 1 (lvl10) or 2 (lvl20). subset != 0 only for level 1 with a
+ * subset key (lvl1 key starts with the lvl0 key): rows exist only for
+ * j >= n0, stored from row 0 (tfhe_ksk_subset_words() words); the subset
+ * key switch copies coefficients j < n0 instead. */
+int tfhe_ksk_gen(int domain_level, uint64_t seed, const void* key_domain,
+                 const uint32_t* key_lvl0, int subset, uint32_t* ksk_out,
+                 size_t ksk_capacity_words);
+
+/* Encrypt count bits as TLWE(+-mu) at the level's fresh noise. Ciphertext i
+ * is drawn from the stream (seed, i) so any subset can be regenerated.
+ * Output stride is the TLWE size of the level (n+1 words). */
+int tfhe_tlwe_encrypt_bits(int level, uint64_t seed, const void* key,
+                           const uint8_t* bits, size_t count, void* ct_out);
+
+/* Phase of count TLWEs as a torus fraction in [-1/2, 1/2). */
+int tfhe_tlwe_phase(int level, const void* key, const void* ct, size_t count,
+                    double* phase_out);
+
+/* Decrypt count TLWEs: bit = phase > 0. */
+int tfhe_tlwe_decrypt_bits(int level, const void* key, const void* ct,
+                           size_t count, uint8_t* bits_out);
+
+/* Words of the structures above (for sizing buffers). */
+size_t tfhe_tlwe_words(int level);
+size_t tfhe_bk_words(int target_level, int key_bundle);
+size_t tfhe_ksk_words(int domain_level);
+size_t tfhe_ksk_subset_words(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TFHE_HOST_H */
