@@ -49,7 +49,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--jobs-per-group", type=int, default=0,
-                    help="bootstraps per workgroup; 0 = autotune over {1, 2, 4}")
+                    help="bootstraps per workgroup; 0 = autotune over {1, 2, 4, 8}")
     ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--plain-streams", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -88,7 +88,7 @@ def main():
                     np.uint8)
 
     # ---- autotune jobs per workgroup
-    cands = [args.jobs_per_group] if args.jobs_per_group else [1, 2, 4]
+    cands = [args.jobs_per_group] if args.jobs_per_group else [1, 2, 4, 8]
     tune = {}
     for c in cands:
         plan = C.Plan(gates, dpool, jobs_per_group=c)

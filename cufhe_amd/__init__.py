@@ -18,7 +18,7 @@ __all__ = [
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libcufhe_amd.so")
+LIB_PATH = os.environ.get("CUFHE_AMD_LIB") or os.path.join(_HERE, "libcufhe_amd.so")
 
 N0, N1, N2 = 636, 1024, 2048
 TLWE0_WORDS, TLWE1_WORDS, TLWE2_WORDS = N0 + 1, N1 + 1, N2 + 1

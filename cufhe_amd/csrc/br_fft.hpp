@@ -23,6 +23,11 @@
 
 namespace cufhe_amd {
 
+// Registers of key values kept in flight ahead of the multiply-accumulate.
+#ifndef CUFHE_BR_PREFETCH
+#define CUFHE_BR_PREFETCH 2
+#endif
+
 template <class P>
 inline constexpr size_t kTrgswFourier =
     static_cast<size_t>(TFHEpp::BootstrappingTRGSWRows<P>) * (P::k + 1) * (P::n / 2);
@@ -139,13 +144,13 @@ __device__ __forceinline__ typename P::T rotated_coef(const typename P::T* p, in
     return neg ? static_cast<T>(0 - v) : v;
 }
 
-// LDS shape of one wave: NB spectrum buffers of H complex (lvl1: one per
-// gadget digit; lvl2: one, digits processed one pass at a time).
+// LDS shape of one wave: one spectrum buffer of H complex; the gadget
+// digits are processed one pass at a time.
 template <class P>
 struct BrShape {
     static constexpr int N = P::n, H = N / 2, R = H / 64, L = P::l;
-    static constexpr int NB = (R == 8) ? L : 1;
-    static constexpr int kPasses = L / NB;
+    static constexpr int NB = 1;
+    static constexpr int kPasses = L;
 };
 
 // One external-product step of the blind rotation for the wave owning
@@ -210,28 +215,54 @@ __device__ __forceinline__ void br_step_fft(AccStore<P>& acc, int j, uint32_t a0
 
 #pragma unroll 1
     for (int ps = 0; ps < S::kPasses; ps++) {
-#pragma unroll
-        for (int qq = 0; qq < NB; qq++) {
-            const int q = ps * NB + qq;
+        {
+            const int qd = ps * NB;
             cplx z[R];
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                z[r] = cplx{static_cast<double>(Gadget<P>::digit(src_at(r), q)),
-                            static_cast<double>(Gadget<P>::digit(src_at(R + r), q))};
+                z[r] = cplx{static_cast<double>(Gadget<P>::digit(src_at(r), qd)),
+                            static_cast<double>(Gadget<P>::digit(src_at(R + r), qd))};
             }
-            cplx* buf = my_spec + qq * H;
-            fft_forward<H>(z, buf, tb);
+            fft_forward<H>(z, my_spec, tb);
 #pragma unroll
-            for (int d = 0; d < R; d++) lds_store(&buf[d * 64 + lane], z[d]);
+            for (int d = 0; d < R; d++) lds_store(&my_spec[d * 64 + lane], z[d]);
         }
+        // Key loads of this pass: column j of rows (comp * L + q), comp = 0, 1.
+        // They do not depend on the spectra, so the first PD registers' worth
+        // is issued before the pair barrier and the rest PD registers ahead
+        // (L2 latency is hidden by loads in flight, not by occupancy).
+        struct KeyVals {
+            cplx k[2][KB ? 3 : 1];
+        };
+        constexpr int PD = KB ? CUFHE_BR_PREFETCH : R;
+        const int q = ps * NB;  // NB == 1: one digit per pass
+        auto load_keys = [&](int d) -> KeyVals {
+            KeyVals v;
+#pragma unroll
+            for (int comp = 0; comp < 2; comp++) {
+                const cplx* kp = col0 + static_cast<size_t>((comp * L + q) * 2) * H + d * 64;
+                v.k[comp][0] = kp[0];
+                if constexpr (KB) {
+                    v.k[comp][1] = kp[TS];
+                    v.k[comp][2] = kp[2 * TS];
+                }
+            }
+            return v;
+        };
+        KeyVals kv[PD];
+#pragma unroll
+        for (int d = 0; d < PD; d++) kv[d] = load_keys(d);
         __syncthreads();
         if constexpr (KB) {
             x0g.pin();
             x1g.pin();
             x01g.pin();
         }
+        const double hq = ldexp(1.0, -static_cast<int>((q + 1) * P::Bgbit));
 #pragma unroll
         for (int d = 0; d < R; d++) {
+            const KeyVals cur = kv[d % PD];
+            if (d + PD < R) kv[d % PD] = load_keys(d + PD);
             cplx x0, x1, x01;
             if constexpr (KB) {
                 x0 = x0g.at(d);
@@ -242,25 +273,18 @@ __device__ __forceinline__ void br_step_fft(AccStore<P>& acc, int j, uint32_t a0
 #pragma unroll
             for (int comp = 0; comp < 2; comp++) {
                 const cplx* spec = comp == j ? my_spec : partner_spec;
-#pragma unroll
-                for (int qq = 0; qq < NB; qq++) {
-                    const int q = ps * NB + qq;
-                    const int row = comp * L + q;
-                    const cplx D = lds_load(&spec[qq * H + d * 64 + lane]);
-                    const cplx* kp = col0 + static_cast<size_t>(row * 2) * H + d * 64;
-                    cplx kv;
-                    if constexpr (KB) {
-                        kv = {comp == j ? ldexp(1.0, -static_cast<int>((q + 1) * P::Bgbit)) : 0.0,
-                              0.0};
-                        cmac(kv, kp[2 * TS], x1);
-                        cmac(kv, kp[TS], x0);
-                        cmac(kv, kp[0], x01);
-                    }
-                    else {
-                        kv = kp[0];
-                    }
-                    cmac(s, D, kv);
+                const cplx D = lds_load(&spec[d * 64 + lane]);
+                cplx k;
+                if constexpr (KB) {
+                    k = {comp == j ? hq : 0.0, 0.0};
+                    cmac(k, cur.k[comp][2], x1);
+                    cmac(k, cur.k[comp][1], x0);
+                    cmac(k, cur.k[comp][0], x01);
                 }
+                else {
+                    k = cur.k[comp][0];
+                }
+                cmac(s, D, k);
             }
             accj[d] = s;
         }

@@ -8,6 +8,12 @@ namespace cufhe_amd {
 
 constexpr int kTlwe0Pad = 640;  // lvl0 TLWE (637 words) padded in LDS
 
+// Occupancy target of the batched blind rotation (waves per SIMD); bounds
+// the VGPR budget the compiler schedules for (3 -> 168 VGPRs).
+#ifndef CUFHE_BR_WAVES_PER_SIMD
+#define CUFHE_BR_WAVES_PER_SIMD 3
+#endif
+
 // ---------------------------------------------------------------------------
 // Polynomial -> Fourier spectra. One wave per polynomial.
 // Reference src/bootstrap_gpu.cu:59-112 (__TRGSW2FFT__): signed torus value
@@ -88,7 +94,7 @@ __global__ __launch_bounds__(256) void k_from_fourier(double* out, const cplx* i
 // ---------------------------------------------------------------------------
 // Batched blind rotation: C jobs per workgroup, two waves per job.
 template <class P, bool KB, int C>
-__global__ __launch_bounds__(128 * C, 3) void k_br_fft(const BrJob* __restrict__ jobs, int njobs,
+__global__ __launch_bounds__(128 * C, CUFHE_BR_WAVES_PER_SIMD) void k_br_fft(const BrJob* __restrict__ jobs, int njobs,
                                                      typename P::T mu, FourierKeys<P> K)
 {
     using T = typename P::T;
@@ -328,11 +334,14 @@ void launch_br_fft(const BrJob* jobs, int njobs, int c, typename P::T mu,
                                    K);
             break;
         case 4:
-            if constexpr (sizeof(typename P::T) == 4 && KB)
+            if constexpr (sizeof(typename P::T) == 4)
                 hipLaunchKernelGGL((k_br_fft<P, KB, 4>), grid, block, 0, st, jobs, njobs, mu,
                                    K);
             break;
         case 8:
+            if constexpr (sizeof(typename P::T) == 4)
+                hipLaunchKernelGGL((k_br_fft<P, KB, 8>), grid, block, 0, st, jobs, njobs, mu,
+                                   K);
             break;
         default:
             break;

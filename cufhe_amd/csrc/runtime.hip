@@ -280,12 +280,13 @@ int plan_launch(cufhe_plan* p, hipStream_t st)
 int pick_c(int c, bool kb, int nbr)
 {
     if (c == 0) {
-        // autotuned default (bench sweep of {1, 2, 4} on MI355X)
-        c = kb ? 2 : 2;
-        if (nbr < 512) c = 1;
+        // default from the MI355X sweep of {1, 2, 4, 8} (lvl1, 4096 gates):
+        // one job per workgroup wins once the key stream is L2-resident
+        c = 1;
+        (void)nbr;
     }
-    if (c != 1 && c != 2 && c != 4) return -1;
-    if (!kb && c > 2) return -1;
+    if (c != 1 && c != 2 && c != 4 && c != 8) return -1;
+    (void)kb;
     return c;
 }
 
@@ -536,7 +537,7 @@ int cufhe_plan_create_lvl1(const cufhe_gate_desc* gates, int n, uint32_t* d_pool
         }
     }
     const int c = pick_c(jobs_per_group, s.k1.kb, nks);
-    if (c < 0) return fail(CUFHE_ERR_UNSUPPORTED, "jobs_per_group must be 0, 1, 2 or 4 (KB) / 1, 2");
+    if (c < 0) return fail(CUFHE_ERR_UNSUPPORTED, "jobs_per_group must be 0, 1, 2, 4 or 8");
     cufhe_plan* p = new cufhe_plan();
     p->c = c;
     p->use_graph = use_graph != 0;
