@@ -55,6 +55,13 @@ def build(jobs=4, verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # C++ drop-in gate test (tests/cpp), linked against the library in-tree
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "test_gate_gpu.cpp")
+    if os.path.exists(src):
+        exe = os.path.join(root, "tests", "cpp", "test_gate_gpu.bin")
+        _run([HIPCC, "-O2", "-std=c++17", "-I", os.path.join(root, "include"), src, "-o", exe,
+              "-L", HERE, "-lcufhe_amd", "-Wl,-rpath,$ORIGIN/../../cufhe_amd"])
     return LIB
 
 

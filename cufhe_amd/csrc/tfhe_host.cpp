@@ -361,6 +361,29 @@ int tfhe_tlwe_encrypt_bits(int level, uint64_t seed, const void* key, const uint
     }
 }
 
+int tfhe_tlwe_encrypt(int level, uint64_t seed, uint64_t index, const void* key, uint64_t m,
+                      double alpha, void* ct)
+{
+    if (!key || !ct) return CUFHE_ERR_ARG;
+    ChaCha20 rng(seed, kTagEnc | index);
+    switch (level) {
+        case 0:
+            tlwe_encrypt<lvl0param>(static_cast<uint32_t*>(ct), static_cast<uint32_t>(m),
+                                    static_cast<const uint32_t*>(key), alpha, rng);
+            return CUFHE_OK;
+        case 1:
+            tlwe_encrypt<lvl1param>(static_cast<uint32_t*>(ct), static_cast<uint32_t>(m),
+                                    static_cast<const uint32_t*>(key), alpha, rng);
+            return CUFHE_OK;
+        case 2:
+            tlwe_encrypt<lvl2param>(static_cast<uint64_t*>(ct), m,
+                                    static_cast<const uint64_t*>(key), alpha, rng);
+            return CUFHE_OK;
+        default:
+            return CUFHE_ERR_ARG;
+    }
+}
+
 int tfhe_tlwe_phase(int level, const void* key, const void* ct, size_t count, double* out)
 {
     if (!key || (!ct && count) || (!out && count)) return CUFHE_ERR_ARG;

@@ -1,0 +1,257 @@
+// C++ drop-in for cuFHEpp's public gate API (reference
+// include/cufhe_gpu.cuh:40-358, src/cufhe_gates_gpu.cu:34-977): the same
+// names (SetGPUNum, Initialize, Ctxt, Stream, Nand ... NMux, g* variants,
+// *_lvl02 gates) and the same argument meaning, implemented over the C ABI of
+// libcufhe_amd.so (cufhe_amd.h). Errors print and exit(-1) as the reference's
+// CuSafeCall does (include/error_gpu.cuh:30-66).
+//
+// Differences a caller can see: one GPU per process (SetGPUNum(n > 1) is
+// refused), and lvl0-input gates (the BR-IKS lvl01 form) are not provided.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <type_traits>
+#include <vector>
+
+#include "cufhe_amd.h"
+#include "tfhe_host.hpp"
+
+namespace cufhe {
+
+inline int _gpuNum = 1;
+inline int streamCount = 0;
+
+inline void CuCheck(int rc, const char* what)
+{
+    if (rc != CUFHE_OK) {
+        std::fprintf(stderr, "cufhe: %s failed (%d): %s\n", what, rc, cufhe_last_error());
+        std::exit(-1);
+    }
+}
+
+inline void SetGPUNum(int gpuNum)
+{
+    CuCheck(cufhe_set_gpu_num(gpuNum), "SetGPUNum");
+    _gpuNum = gpuNum;
+}
+
+inline void Initialize() { CuCheck(cufhe_initialize(), "Initialize"); }
+
+inline void Initialize(const TFHEpp::EvalKey& ek)
+{
+    if (!ek.bklvl01 || !ek.iksklvl10) {
+        std::fprintf(stderr, "cufhe: Initialize(ek) needs bklvl01 and iksklvl10\n");
+        std::exit(-1);
+    }
+    CuCheck(cufhe_initialize_lvl01(ek.bklvl01->data(), ek.bklvl01->size(), ek.key_bundle,
+                                   ek.iksklvl10->data(), ek.iksklvl10->size()),
+            "Initialize");
+}
+
+inline void Initialize_lvl02(const TFHEpp::EvalKey& ek, const TFHEpp::SecretKey&)
+{
+    if (!ek.bklvl02 || !ek.iksklvl20) {
+        std::fprintf(stderr, "cufhe: Initialize_lvl02 needs bklvl02 and iksklvl20\n");
+        std::exit(-1);
+    }
+    CuCheck(cufhe_initialize_lvl02(ek.bklvl02->data(), ek.bklvl02->size(), ek.key_bundle,
+                                   ek.iksklvl20->data(), ek.iksklvl20->size()),
+            "Initialize_lvl02");
+}
+
+inline void CleanUp() { CuCheck(cufhe_cleanup(), "CleanUp"); }
+inline void CleanUp_lvl02() { CuCheck(cufhe_cleanup_lvl02(), "CleanUp_lvl02"); }
+inline void Synchronize() { CuCheck(cufhe_synchronize(), "Synchronize"); }
+
+/** Ciphertext: host TLWE plus one device copy (reference cufhe_gpu.cuh:100-121). */
+template <class P>
+struct Ctxt {
+    Ctxt()
+    {
+        tlwedevices.resize(_gpuNum);
+        void* p = nullptr;
+        CuCheck(cufhe_device_alloc(&p, sizeof(tlwehost)), "Ctxt alloc");
+        tlwedevices[0] = static_cast<typename P::T*>(p);
+    }
+    ~Ctxt()
+    {
+        for (auto* p : tlwedevices) cufhe_device_free(p);
+    }
+    Ctxt(const Ctxt&) = delete;
+    Ctxt& operator=(const Ctxt&) = delete;
+
+    alignas(64) TFHEpp::TLWE<P> tlwehost;
+    std::vector<typename P::T*> tlwedevices;
+};
+
+/** Stream handle (reference cufhe_gpu.cuh:162-199). */
+class Stream {
+   public:
+    Stream() : st_(nullptr), device_id_(streamCount % _gpuNum) { streamCount++; }
+    explicit Stream(int device_id) : st_(nullptr), device_id_(device_id) { streamCount++; }
+    void Create()
+    {
+        void* s = nullptr;
+        CuCheck(cufhe_stream_create(&s), "Stream::Create");
+        st_ = static_cast<hipStream_t>(s);
+    }
+    void Destroy()
+    {
+        CuCheck(cufhe_stream_destroy(st_), "Stream::Destroy");
+        st_ = nullptr;
+    }
+    hipStream_t st() { return st_; }
+    int device_id() { return device_id_; }
+
+   private:
+    hipStream_t st_;
+    int device_id_;
+};
+
+inline bool StreamQuery(Stream st)
+{
+    const int r = cufhe_stream_query(st.st());
+    CuCheck(r < 0 ? r : CUFHE_OK, "StreamQuery");
+    return r == 1;
+}
+
+template <class P>
+inline void CtxtCopyH2D(Ctxt<P>& c, Stream st)
+{
+    CuCheck(cufhe_memcpy_h2d(c.tlwedevices[0], c.tlwehost.data(), sizeof(c.tlwehost), st.st()),
+            "CtxtCopyH2D");
+}
+
+template <class P>
+inline void CtxtCopyD2H(Ctxt<P>& c, Stream st)
+{
+    CuCheck(cufhe_memcpy_d2h(c.tlwehost.data(), c.tlwedevices[0], sizeof(c.tlwehost), st.st()),
+            "CtxtCopyD2H");
+}
+
+template <class P>
+void CopyOnHost(Ctxt<P>& out, Ctxt<P>& in)
+{
+    out.tlwehost = in.tlwehost;
+}
+
+namespace detail {
+template <class P>
+inline void gate(int op, Ctxt<P>& out, Ctxt<P>* in0, Ctxt<P>* in1, Ctxt<P>* inc, Stream st)
+{
+    static_assert(std::is_same_v<P, TFHEpp::lvl1param>,
+                  "the lvl1 gate API takes Ctxt<lvl1param>; lvl0 gates are the *_lvl02 functions");
+    CuCheck(cufhe_gate_lvl1(op, out.tlwedevices[0], in0 ? in0->tlwedevices[0] : nullptr,
+                            in1 ? in1->tlwedevices[0] : nullptr, inc ? inc->tlwedevices[0] : nullptr,
+                            st.st()),
+            "gate");
+}
+template <class P>
+inline void host_gate(int op, Ctxt<P>& out, Ctxt<P>* in0, Ctxt<P>* in1, Ctxt<P>* inc, Stream st)
+{
+    for (Ctxt<P>* c : {in0, in1, inc})
+        if (c) CtxtCopyH2D<P>(*c, st);
+    gate<P>(op, out, in0, in1, inc, st);
+    CtxtCopyD2H<P>(out, st);
+}
+inline void gate02(int op, Ctxt<TFHEpp::lvl0param>& out, Ctxt<TFHEpp::lvl0param>* in0,
+                   Ctxt<TFHEpp::lvl0param>* in1, Ctxt<TFHEpp::lvl0param>* inc, Stream st)
+{
+    using C0 = Ctxt<TFHEpp::lvl0param>;
+    for (C0* c : {in0, in1, inc})
+        if (c) CtxtCopyH2D<TFHEpp::lvl0param>(*c, st);
+    CuCheck(cufhe_gate_lvl02(op, out.tlwedevices[0], in0 ? in0->tlwedevices[0] : nullptr,
+                             in1 ? in1->tlwedevices[0] : nullptr,
+                             inc ? inc->tlwedevices[0] : nullptr, st.st()),
+            "gate_lvl02");
+    CtxtCopyD2H<TFHEpp::lvl0param>(out, st);
+}
+}  // namespace detail
+
+// Host-buffer gates: copy inputs H2D, bootstrap, copy the result D2H (all
+// asynchronous on st; results valid after Synchronize()).
+#define CUFHE_BINARY_GATE(Name, Code)                                          \
+    template <class P>                                                         \
+    void Name(Ctxt<P>& out, Ctxt<P>& in0, Ctxt<P>& in1, Stream st)            \
+    {                                                                          \
+        detail::host_gate<P>(Code, out, &in0, &in1, nullptr, st);              \
+    }                                                                          \
+    template <class P>                                                         \
+    void g##Name(Ctxt<P>& out, Ctxt<P>& in0, Ctxt<P>& in1, Stream st)         \
+    {                                                                          \
+        detail::gate<P>(Code, out, &in0, &in1, nullptr, st);                   \
+    }                                                                          \
+    inline void Name##_lvl02(Ctxt<TFHEpp::lvl0param>& out, Ctxt<TFHEpp::lvl0param>& in0, \
+                             Ctxt<TFHEpp::lvl0param>& in1, Stream st)          \
+    {                                                                          \
+        detail::gate02(Code, out, &in0, &in1, nullptr, st);                    \
+    }
+CUFHE_BINARY_GATE(Nand, CUFHE_NAND)
+CUFHE_BINARY_GATE(Or, CUFHE_OR)
+CUFHE_BINARY_GATE(OrYN, CUFHE_ORYN)
+CUFHE_BINARY_GATE(OrNY, CUFHE_ORNY)
+CUFHE_BINARY_GATE(And, CUFHE_AND)
+CUFHE_BINARY_GATE(AndYN, CUFHE_ANDYN)
+CUFHE_BINARY_GATE(AndNY, CUFHE_ANDNY)
+CUFHE_BINARY_GATE(Nor, CUFHE_NOR)
+CUFHE_BINARY_GATE(Xor, CUFHE_XOR)
+CUFHE_BINARY_GATE(Xnor, CUFHE_XNOR)
+#undef CUFHE_BINARY_GATE
+
+// Mux(out, inc, in1, in0): out = inc ? in1 : in0 (reference argument order).
+template <class P>
+void Mux(Ctxt<P>& out, Ctxt<P>& inc, Ctxt<P>& in1, Ctxt<P>& in0, Stream st)
+{
+    detail::host_gate<P>(CUFHE_MUX, out, &in0, &in1, &inc, st);
+}
+template <class P>
+void gMux(Ctxt<P>& out, Ctxt<P>& inc, Ctxt<P>& in1, Ctxt<P>& in0, Stream st)
+{
+    detail::gate<P>(CUFHE_MUX, out, &in0, &in1, &inc, st);
+}
+template <class P>
+void NMux(Ctxt<P>& out, Ctxt<P>& inc, Ctxt<P>& in1, Ctxt<P>& in0, Stream st)
+{
+    detail::host_gate<P>(CUFHE_NMUX, out, &in0, &in1, &inc, st);
+}
+template <class P>
+void gNMux(Ctxt<P>& out, Ctxt<P>& inc, Ctxt<P>& in1, Ctxt<P>& in0, Stream st)
+{
+    detail::gate<P>(CUFHE_NMUX, out, &in0, &in1, &inc, st);
+}
+inline void Mux_lvl02(Ctxt<TFHEpp::lvl0param>& out, Ctxt<TFHEpp::lvl0param>& inc,
+                      Ctxt<TFHEpp::lvl0param>& in1, Ctxt<TFHEpp::lvl0param>& in0, Stream st)
+{
+    detail::gate02(CUFHE_MUX, out, &in0, &in1, &inc, st);
+}
+inline void NMux_lvl02(Ctxt<TFHEpp::lvl0param>& out, Ctxt<TFHEpp::lvl0param>& inc,
+                       Ctxt<TFHEpp::lvl0param>& in1, Ctxt<TFHEpp::lvl0param>& in0, Stream st)
+{
+    detail::gate02(CUFHE_NMUX, out, &in0, &in1, &inc, st);
+}
+
+template <class P>
+void Not(Ctxt<P>& out, Ctxt<P>& in, Stream st)
+{
+    detail::host_gate<P>(CUFHE_NOT, out, &in, nullptr, nullptr, st);
+}
+template <class P>
+void gNot(Ctxt<P>& out, Ctxt<P>& in, Stream st)
+{
+    detail::gate<P>(CUFHE_NOT, out, &in, nullptr, nullptr, st);
+}
+template <class P>
+void Copy(Ctxt<P>& out, Ctxt<P>& in, Stream st)
+{
+    detail::host_gate<P>(CUFHE_COPY, out, &in, nullptr, nullptr, st);
+}
+template <class P>
+void gCopy(Ctxt<P>& out, Ctxt<P>& in, Stream st)
+{
+    detail::gate<P>(CUFHE_COPY, out, &in, nullptr, nullptr, st);
+}
+
+}  // namespace cufhe

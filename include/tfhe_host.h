@@ -55,6 +55,11 @@ int tfhe_ksk_gen(int domain_level, uint64_t seed, const void* key_domain,
 int tfhe_tlwe_encrypt_bits(int level, uint64_t seed, const void* key,
                            const uint8_t* bits, size_t count, void* ct_out);
 
+/* One TLWE encryption of the torus word m (uint32 for levels 0/1, uint64
+ * for level 2) with noise alpha, drawn from the stream (seed, index). */
+int tfhe_tlwe_encrypt(int level, uint64_t seed, uint64_t index, const void* key, uint64_t m,
+                      double alpha, void* ct_out);
+
 /* Phase of count TLWEs as a torus fraction in [-1/2, 1/2). */
 int tfhe_tlwe_phase(int level, const void* key, const void* ct, size_t count,
                     double* phase_out);

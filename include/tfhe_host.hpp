@@ -1,0 +1,135 @@
+// C++ host layer with TFHEpp's names (SecretKey, EvalKey, tlweSymEncrypt,
+// tlweSymDecrypt) over the C ABI in tfhe_host.h. It stands in for the
+// un-vendored TFHEpp submodule the reference builds against (reference
+// .gitmodules:1-3); keys and ciphertexts use TFHEpp's layouts, so code written
+// against TFHEpp's EvalKey/TLWE types compiles against this header.
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <memory>
+#include <random>
+#include <type_traits>
+#include <vector>
+
+#include "tfhe_host.h"
+#include "tfhe_params.hpp"
+
+namespace TFHEpp {
+
+namespace detail {
+inline void check(int rc, const char* what)
+{
+    if (rc != 0) {
+        std::fprintf(stderr, "TFHEpp host layer: %s failed (%d)\n", what, rc);
+        std::exit(-1);
+    }
+}
+template <class P>
+constexpr int level()
+{
+    if constexpr (std::is_same_v<P, lvl0param>) return 0;
+    if constexpr (std::is_same_v<P, lvl1param>) return 1;
+    if constexpr (std::is_same_v<P, lvl2param>) return 2;
+    return -1;
+}
+inline uint64_t& encrypt_counter()
+{
+    static uint64_t c = 0;
+    return c;
+}
+inline uint64_t& encrypt_seed()
+{
+    static uint64_t s = std::random_device{}() | (static_cast<uint64_t>(std::random_device{}()) << 32);
+    return s;
+}
+}  // namespace detail
+
+struct SecretKey {
+    struct {
+        Key<lvl0param> lvl0;
+        Key<lvl1param> lvl1;
+        Key<lvl2param> lvl2;
+        template <class P>
+        const Key<P>& get() const
+        {
+            if constexpr (std::is_same_v<P, lvl0param>) return lvl0;
+            else if constexpr (std::is_same_v<P, lvl1param>) return lvl1;
+            else return lvl2;
+        }
+    } key;
+    uint64_t seed;
+    explicit SecretKey(uint64_t s) : seed(s)
+    {
+        detail::check(tfhe_secret_key_gen(s, 0, key.lvl0.data(), key.lvl1.data(), key.lvl2.data()),
+                      "tfhe_secret_key_gen");
+    }
+    SecretKey() : SecretKey(std::random_device{}() | (static_cast<uint64_t>(std::random_device{}()) << 32)) {}
+};
+
+// Bootstrapping keys (polynomial domain) and key-switching keys.
+struct EvalKey {
+    bool key_bundle = true;  // 2-bit key bundle (reference default build)
+    std::unique_ptr<std::vector<uint32_t>> bklvl01;
+    std::unique_ptr<std::vector<uint64_t>> bklvl02;
+    std::unique_ptr<std::vector<uint32_t>> iksklvl10;
+    std::unique_ptr<std::vector<uint32_t>> iksklvl20;
+
+    template <class P>
+    void emplacebk(const SecretKey& sk)
+    {
+        if constexpr (std::is_same_v<P, lvl01param>) {
+            bklvl01 = std::make_unique<std::vector<uint32_t>>(tfhe_bk_words(1, key_bundle));
+            detail::check(tfhe_bk_gen(1, key_bundle, sk.seed ^ 0xB01, sk.key.lvl0.data(),
+                                      sk.key.lvl1.data(), bklvl01->data(), bklvl01->size(), nullptr),
+                          "tfhe_bk_gen");
+        }
+        else {
+            static_assert(std::is_same_v<P, lvl02param>, "unsupported bootstrapping key");
+            bklvl02 = std::make_unique<std::vector<uint64_t>>(tfhe_bk_words(2, key_bundle));
+            detail::check(tfhe_bk_gen(2, key_bundle, sk.seed ^ 0xB02, sk.key.lvl0.data(),
+                                      sk.key.lvl2.data(), bklvl02->data(), bklvl02->size(), nullptr),
+                          "tfhe_bk_gen");
+        }
+    }
+    template <class P>
+    void emplaceiksk(const SecretKey& sk)
+    {
+        if constexpr (std::is_same_v<P, lvl10param>) {
+            iksklvl10 = std::make_unique<std::vector<uint32_t>>(tfhe_ksk_words(1));
+            detail::check(tfhe_ksk_gen(1, sk.seed ^ 0x510, sk.key.lvl1.data(), sk.key.lvl0.data(), 0,
+                                       iksklvl10->data(), iksklvl10->size()),
+                          "tfhe_ksk_gen");
+        }
+        else {
+            static_assert(std::is_same_v<P, lvl20param>, "unsupported key-switching key");
+            iksklvl20 = std::make_unique<std::vector<uint32_t>>(tfhe_ksk_words(2));
+            detail::check(tfhe_ksk_gen(2, sk.seed ^ 0x520, sk.key.lvl2.data(), sk.key.lvl0.data(), 0,
+                                       iksklvl20->data(), iksklvl20->size()),
+                          "tfhe_ksk_gen");
+        }
+    }
+};
+
+template <class P>
+TLWE<P> tlweSymEncrypt(const typename P::T p, const double alpha, const Key<P>& key)
+{
+    TLWE<P> c;
+    detail::check(tfhe_tlwe_encrypt(detail::level<P>(), detail::encrypt_seed(),
+                                    detail::encrypt_counter()++, key.data(), static_cast<uint64_t>(p),
+                                    alpha, c.data()),
+                  "tfhe_tlwe_encrypt");
+    return c;
+}
+
+template <class P>
+bool tlweSymDecrypt(const TLWE<P>& c, const Key<P>& key)
+{
+    uint8_t bit = 0;
+    detail::check(tfhe_tlwe_decrypt_bits(detail::level<P>(), key.data(), c.data(), 1, &bit),
+                  "tfhe_tlwe_decrypt_bits");
+    return bit != 0;
+}
+
+}  // namespace TFHEpp
