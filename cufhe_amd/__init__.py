@@ -95,6 +95,7 @@ def lib():
         "cufhe_gate_lvl02": ([i32, vp, vp, vp, vp, vp], i32),
         "cufhe_plan_create_lvl1": ([vp, i32, vp, i32, i32, ctypes.POINTER(vp)], i32),
         "cufhe_plan_run": ([vp, vp], i32),
+        "cufhe_plan_create_lvl02": ([vp, i32, vp, i32, ctypes.POINTER(vp)], i32),
         "cufhe_plan_destroy": ([vp], i32),
         "cufhe_plan_bootstraps": ([vp], i32),
         "cufhe_gate_batch_lvl1": ([vp, i32, vp, i32, vp], i32),
@@ -261,16 +262,21 @@ def _sp(stream):
 class Plan:
     """Batched gates over a device pool of lvl1 TLWEs."""
 
-    def __init__(self, gates, pool, jobs_per_group=0, use_graph=False):
+    def __init__(self, gates, pool, jobs_per_group=0, use_graph=False, level=1):
         arr = (GateDesc * len(gates))()
         for i, g in enumerate(gates):
             op, out, in0, in1, inc = g
             arr[i] = GateDesc(GATES[op] if isinstance(op, str) else int(op), out, in0, in1, inc)
         self._arr = arr
         h = ctypes.c_void_p()
-        check(lib().cufhe_plan_create_lvl1(ctypes.cast(arr, ctypes.c_void_p), len(gates),
-                                           pool.ptr, int(jobs_per_group), int(use_graph),
-                                           ctypes.byref(h)), "cufhe_plan_create_lvl1")
+        if level == 2:
+            check(lib().cufhe_plan_create_lvl02(ctypes.cast(arr, ctypes.c_void_p), len(gates),
+                                                pool.ptr, int(use_graph), ctypes.byref(h)),
+                  "cufhe_plan_create_lvl02")
+        else:
+            check(lib().cufhe_plan_create_lvl1(ctypes.cast(arr, ctypes.c_void_p), len(gates),
+                                               pool.ptr, int(jobs_per_group), int(use_graph),
+                                               ctypes.byref(h)), "cufhe_plan_create_lvl1")
         self.handle = h
 
     @property

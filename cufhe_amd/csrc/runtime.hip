@@ -247,6 +247,7 @@ int alloc_upload(T** dst, const std::vector<T>& v)
 using namespace cufhe_amd;
 
 struct cufhe_plan {
+    int level = 1;  // 1: lvl1 gates (IKS-BR); 2: lvl02 gates (BR-IKS)
     int njobs_ks = 0, njobs_br = 0, njobs_lin = 0;
     int c = 2;
     bool use_graph = false;
@@ -264,6 +265,21 @@ namespace {
 int plan_launch(cufhe_plan* p, hipStream_t st)
 {
     State& s = state();
+    if (p->level == 2) {
+        // BR-IKS order: rotations (pre-added lvl0 inputs) -> lvl2 TLWEs ->
+        // key switch (MUX halves combined in its pre-add)
+        if (p->njobs_br) {
+            if (s.k2.kb)
+                launch_br_fft<lvl2param, true>(p->br, p->njobs_br, 1,
+                                               static_cast<uint64_t>(lvl2param::μ), s.k2.fk, st);
+            else
+                launch_br_fft<lvl2param, false>(p->br, p->njobs_br, 1,
+                                                static_cast<uint64_t>(lvl2param::μ), s.k2.fk, st);
+        }
+        if (p->njobs_ks) launch_ks_batch(2, false, p->ks, p->njobs_ks, s.k2.ksk, st);
+        if (p->njobs_lin) launch_linear(p->lin, p->njobs_lin, st);
+        return check_launch("batched lvl02 gate launch");
+    }
     if (p->njobs_ks) launch_ks_batch(1, false, p->ks, p->njobs_ks, s.k1.ksk, st);
     if (p->njobs_br) {
         if (s.k1.kb)
@@ -600,11 +616,86 @@ int cufhe_plan_create_lvl1(const cufhe_gate_desc* gates, int n, uint32_t* d_pool
     return CUFHE_OK;
 }
 
+int cufhe_plan_create_lvl02(const cufhe_gate_desc* gates, int n, uint32_t* d_pool,
+                            int use_graph, cufhe_plan** out)
+{
+    State& s = state();
+    if (!out) return fail(CUFHE_ERR_ARG, "null plan handle");
+    *out = nullptr;
+    if (!s.k2.loaded) return fail(CUFHE_ERR_STATE, "lvl02 keys not loaded (cufhe_initialize_lvl02)");
+    if (n < 0 || (n > 0 && (!gates || !d_pool))) return fail(CUFHE_ERR_ARG, "bad gate list");
+    constexpr uint32_t W0 = lvl0param::n + 1;   // lvl0 wire stride
+    constexpr uint32_t W2 = lvl2param::n + 1;   // lvl2 TLWE (uint64 words)
+    constexpr uint32_t mu0 = static_cast<uint32_t>(lvl0param::μ);
+    constexpr uint64_t mu2 = static_cast<uint64_t>(lvl2param::μ);
+    int nbr = 0;
+    for (int i = 0; i < n; i++) {
+        const int op = gates[i].op;
+        if (op < 0 || op >= CUFHE_GATE_COUNT) return fail(CUFHE_ERR_ARG, "unknown gate op");
+        if (op == CUFHE_MUX || op == CUFHE_NMUX)
+            nbr += 2;
+        else if (op != CUFHE_NOT && op != CUFHE_COPY)
+            nbr += 1;
+    }
+    cufhe_plan* p = new cufhe_plan();
+    p->level = 2;
+    p->c = 1;
+    p->use_graph = use_graph != 0;
+    if (nbr) {
+        if (hipMalloc(reinterpret_cast<void**>(&p->ws),
+                      static_cast<size_t>(nbr) * W2 * sizeof(uint64_t)) != hipSuccess) {
+            delete p;
+            return fail(CUFHE_ERR_NOMEM, "plan workspace allocation failed");
+        }
+    }
+    uint64_t* tl2 = reinterpret_cast<uint64_t*>(p->ws);
+    std::vector<KsJob> ks;
+    std::vector<BrJob> br;
+    std::vector<LinJob> lin;
+    auto wire = [&](uint32_t w) { return d_pool + static_cast<size_t>(w) * W0; };
+    for (int i = 0; i < n; i++) {
+        const cufhe_gate_desc& gd = gates[i];
+        if (gd.op == CUFHE_NOT || gd.op == CUFHE_COPY) {
+            lin.push_back(LinJob{wire(gd.in0), nullptr, wire(gd.out), gd.op == CUFHE_NOT ? -1 : 1, 0,
+                                 0u, static_cast<int32_t>(W0)});
+            continue;
+        }
+        if (gd.op == CUFHE_MUX || gd.op == CUFHE_NMUX) {
+            uint64_t* r1 = tl2 + static_cast<size_t>(br.size()) * W2;
+            br.push_back(BrJob{wire(gd.inc), wire(gd.in1), r1, 1, 1, 0u - mu0, 0u});
+            uint64_t* r0 = tl2 + static_cast<size_t>(br.size()) * W2;
+            br.push_back(BrJob{wire(gd.inc), wire(gd.in0), r0, -1, 1, 0u - mu0, 0u});
+            const int sg = gd.op == CUFHE_MUX ? 1 : -1;
+            ks.push_back(KsJob{r1, r0, wire(gd.out), sg, sg, sg > 0 ? mu2 : 0 - mu2});
+            continue;
+        }
+        GateLin gl;
+        gate_lin(gd.op, gl);
+        uint64_t* r = tl2 + static_cast<size_t>(br.size()) * W2;
+        br.push_back(BrJob{wire(gd.in0), wire(gd.in1), r, gl.ca, gl.cb,
+                           static_cast<uint32_t>(gl.off_mu) * mu0, 0u});
+        ks.push_back(KsJob{r, nullptr, wire(gd.out), 1, 0, 0});
+    }
+    int rc = alloc_upload(&p->ks, ks);
+    if (rc == CUFHE_OK) rc = alloc_upload(&p->br, br);
+    if (rc == CUFHE_OK) rc = alloc_upload(&p->lin, lin);
+    if (rc != CUFHE_OK) {
+        cufhe_plan_destroy(p);
+        return rc;
+    }
+    p->njobs_ks = static_cast<int>(ks.size());
+    p->njobs_br = static_cast<int>(br.size());
+    p->njobs_lin = static_cast<int>(lin.size());
+    *out = p;
+    return CUFHE_OK;
+}
+
 int cufhe_plan_run(cufhe_plan* p, void* stream)
 {
     if (!p) return fail(CUFHE_ERR_ARG, "null plan");
     State& s = state();
-    if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded");
+    if (!(p->level == 2 ? s.k2.loaded : s.k1.loaded))
+        return fail(CUFHE_ERR_STATE, "keys of the plan's level not loaded");
     const hipStream_t st = as_stream(stream);
     if (!p->use_graph) return plan_launch(p, st);
     if (!p->exec) {
@@ -646,8 +737,40 @@ int cufhe_plan_run_timed(cufhe_plan* p, void* stream, float* kernel_ms)
 {
     if (!p || !kernel_ms) return fail(CUFHE_ERR_ARG, "null argument");
     State& s = state();
-    if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded");
     const hipStream_t st = as_stream(stream);
+    if (p->level == 2) {
+        if (!s.k2.loaded) return fail(CUFHE_ERR_STATE, "lvl02 keys not loaded");
+        hipEvent_t e[4];
+        for (auto& x : e) CUFHE_HIP_TRY(hipEventCreate(&x));
+        CUFHE_HIP_TRY(hipEventRecord(e[0], st));
+        if (p->njobs_br) {
+            if (s.k2.kb)
+                launch_br_fft<lvl2param, true>(p->br, p->njobs_br, 1,
+                                               static_cast<uint64_t>(lvl2param::μ), s.k2.fk, st);
+            else
+                launch_br_fft<lvl2param, false>(p->br, p->njobs_br, 1,
+                                                static_cast<uint64_t>(lvl2param::μ), s.k2.fk, st);
+        }
+        CUFHE_HIP_TRY(hipEventRecord(e[1], st));
+        if (p->njobs_ks) launch_ks_batch(2, false, p->ks, p->njobs_ks, s.k2.ksk, st);
+        CUFHE_HIP_TRY(hipEventRecord(e[2], st));
+        if (p->njobs_lin) launch_linear(p->lin, p->njobs_lin, st);
+        CUFHE_HIP_TRY(hipEventRecord(e[3], st));
+        int rc2 = check_launch("timed lvl02 plan");
+        if (rc2 == CUFHE_OK) {
+            const hipError_t er = hipEventSynchronize(e[3]);
+            if (er != hipSuccess) rc2 = fail(CUFHE_ERR_HIP, std::string("timed plan: ") + hipGetErrorString(er));
+        }
+        if (rc2 == CUFHE_OK) {
+            // report in the lvl1 order: [0] key switch, [1] blind rotation, [2] combine
+            (void)hipEventElapsedTime(&kernel_ms[1], e[0], e[1]);
+            (void)hipEventElapsedTime(&kernel_ms[0], e[1], e[2]);
+            (void)hipEventElapsedTime(&kernel_ms[2], e[2], e[3]);
+        }
+        for (auto& x : e) (void)hipEventDestroy(x);
+        return rc2;
+    }
+    if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded");
     hipEvent_t ev[4];
     for (auto& e : ev) CUFHE_HIP_TRY(hipEventCreate(&e));
     CUFHE_HIP_TRY(hipEventRecord(ev[0], st));

@@ -133,6 +133,10 @@ int cufhe_gate_lvl02(int op, uint32_t* out, const uint32_t* in0, const uint32_t*
  * graph on the first run and replays it. */
 int cufhe_plan_create_lvl1(const cufhe_gate_desc* gates, int n, uint32_t* d_pool,
                            int jobs_per_group, int use_graph, cufhe_plan** plan);
+/* Plan a batch of lvl02 gates (BR-IKS order) over a device pool of lvl0
+ * TLWEs (wire w at d_pool + 637 w); same semantics as the lvl02 gates. */
+int cufhe_plan_create_lvl02(const cufhe_gate_desc* gates, int n, uint32_t* d_pool,
+                            int use_graph, cufhe_plan** plan);
 int cufhe_plan_run(cufhe_plan* plan, void* stream);
 int cufhe_plan_destroy(cufhe_plan* plan);
 /* Bootstraps (blind rotations) a plan performs per run. */
