@@ -74,6 +74,9 @@ def lib():
         "cufhe_initialize": ([], i32),
         "cufhe_initialize_lvl01": ([vp, sz, i32, vp, sz], i32),
         "cufhe_initialize_lvl02": ([vp, sz, i32, vp, sz], i32),
+        "cufhe_initialize_lvl01_ex": ([vp, sz, i32, i32, vp, sz, i32], i32),
+        "cufhe_ntt_transform": ([vp, vp, i32, i32, vp], i32),
+        "cufhe_ntt_root": ([], u32),
         "cufhe_cleanup": ([], i32),
         "cufhe_cleanup_lvl02": ([], i32),
         "cufhe_synchronize": ([], i32),
@@ -101,6 +104,7 @@ def lib():
         "cufhe_gate_batch_lvl1": ([vp, i32, vp, i32, vp], i32),
         "cufhe_identity_key_switch_lvl10": ([vp, vp, i32, vp], i32),
         "cufhe_bootstrap_lvl01": ([vp, vp, u32, i32, i32, vp], i32),
+        "cufhe_bootstrap_lvl02": ([vp, vp, u64, i32, vp], i32),
         "cufhe_ffnt_forward": ([i32, vp, i32, vp, i32, vp], i32),
         "cufhe_ffnt_inverse": ([i32, vp, vp, i32, vp], i32),
         "cufhe_trgsw_to_fourier": ([i32, vp, vp, i32, vp], i32),
@@ -114,6 +118,7 @@ def lib():
         "tfhe_tlwe_words": ([i32], sz),
         "tfhe_bk_words": ([i32, i32], sz),
         "tfhe_ksk_words": ([i32], sz),
+        "tfhe_ksk_subset_words": ([], sz),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -153,12 +158,26 @@ def bk_gen(level, key_bundle, seed, k0, kt):
     return out
 
 
-def ksk_gen(level, seed, kd, k0):
-    words = lib().tfhe_ksk_words(level)
+def ksk_gen(level, seed, kd, k0, subset=False):
+    """KeySwitchingKey<lvl10/lvl20>; subset=True (level 1 only) keeps the rows
+    j >= n0 of a subset key (the lvl1 key starts with the lvl0 key)."""
+    words = lib().tfhe_ksk_subset_words() if subset else lib().tfhe_ksk_words(level)
     out = np.zeros(words, np.uint32)
-    check(lib().tfhe_ksk_gen(level, seed, _ptr(kd), _ptr(k0), 0, _ptr(out), words),
+    check(lib().tfhe_ksk_gen(level, seed, _ptr(kd), _ptr(k0), int(subset), _ptr(out), words),
           "tfhe_ksk_gen")
     return out
+
+
+BACKEND_FFT, BACKEND_NTT = 0, 1
+BR_PLAIN, BR_KEY_BUNDLE, BR_BLOCK_BINARY = 0, 1, 2
+
+
+def initialize_lvl01(bk, ksk, backend=BACKEND_FFT, variant=BR_KEY_BUNDLE, subset=False):
+    """Load lvl1 gate keys (polynomial-domain BK and KSK) on the GPU."""
+    bk = np.ascontiguousarray(bk, np.uint32)
+    ksk = np.ascontiguousarray(ksk, np.uint32)
+    check(lib().cufhe_initialize_lvl01_ex(_ptr(bk), bk.size, backend, variant, _ptr(ksk), ksk.size,
+                                          int(subset)), "cufhe_initialize_lvl01_ex")
 
 
 def tlwe_words(level):

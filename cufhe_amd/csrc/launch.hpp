@@ -12,8 +12,8 @@ namespace cufhe_amd {
 struct cplx;
 template <class P>
 struct FourierKeys;
-template <class P>
 struct NttKeys;
+struct NttTables;
 
 template <class P>
 void launch_to_fourier(cplx* out, const void* in, int npolys, bool integer,
@@ -41,5 +41,17 @@ void launch_linear_one(const LinJob& job, hipStream_t st);
 template <class P>
 void launch_cmux_fft(typename P::T* trlwe, const cplx* trgsw, uint32_t abar,
                      const FourierKeys<P>& K, hipStream_t st);
+
+// NTT backend (lvl1): mode 0 plain CMUX, 1 key bundle, 2 block-binary.
+void launch_to_ntt(uint32_t* out, const uint32_t* in, int npolys, const NttTables& tb,
+                   hipStream_t st);
+void launch_ntt_natural(bool inverse, uint32_t* out, const uint32_t* in, int npolys,
+                        const NttTables& tb, hipStream_t st);
+void launch_br_ntt(int mode, const BrJob* jobs, int njobs, int c, uint32_t mu, const NttKeys& K,
+                   hipStream_t st);
+void launch_gate_lvl1_ntt(int mode, bool subset, bool mux, uint32_t* out, const uint32_t* in0,
+                          const uint32_t* in1, const uint32_t* inc, int32_t ca, int32_t cb,
+                          uint32_t offset, int32_t mux_sign, const uint32_t* ksk, const NttKeys& K,
+                          hipStream_t st);
 
 }  // namespace cufhe_amd

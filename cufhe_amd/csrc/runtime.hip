@@ -13,6 +13,7 @@
 #include "../../include/cufhe_amd.h"
 #include "../../include/tfhe_params.hpp"
 #include "br_fft.hpp"
+#include "br_ntt.hpp"
 #include "launch.hpp"
 
 namespace cufhe_amd {
@@ -41,13 +42,24 @@ struct TableSet {
     cplx* e2n = nullptr;
 };
 
+// NTT-backend tables (Montgomery form), see ntt_wave.hpp.
+struct NttTableSet {
+    uint32_t* mem = nullptr;  // one allocation holding every table
+    NttTables tb{};
+};
+
 template <class P>
 struct LevelKeys {
     bool loaded = false;
     bool kb = false;
+    int backend = CUFHE_BACKEND_FFT;
+    int variant = CUFHE_BR_PLAIN;
+    bool subset = false;
     cplx* bk = nullptr;
+    uint32_t* bk_ntt = nullptr;
     uint32_t* ksk = nullptr;
     FourierKeys<P> fk{};
+    NttKeys nk{};
 };
 
 struct State {
@@ -56,6 +68,7 @@ struct State {
     int device = 0;
     TableSet<lvl1param> t1;
     TableSet<lvl2param> t2;
+    NttTableSet tn;
     LevelKeys<lvl1param> k1;
     LevelKeys<lvl2param> k2;
 };
@@ -110,6 +123,63 @@ int build_tables(TableSet<P>& ts)
     return CUFHE_OK;
 }
 
+// Host Montgomery helpers for P = 2147473409.
+uint32_t host_mulmod(uint32_t a, uint32_t b)
+{
+    return static_cast<uint32_t>(static_cast<uint64_t>(a) * b % ntt31::P);
+}
+uint32_t host_to_mont(uint32_t x) { return static_cast<uint32_t>((static_cast<uint64_t>(x) << 32) % ntt31::P); }
+// psi^e for any integer e (psi has order 2048)
+uint32_t host_psi_pow(long long e) { return static_cast<uint32_t>(ntt31::psi_pow(e)); }
+
+int build_ntt_tables(NttTableSet& ts)
+{
+    constexpr int N = lvl1param::n;
+    // layout: twl[1024] twli[1024] tw64[64] tw64i[64] cr[16] cri[16] pw[2048]
+    std::vector<uint32_t> h(1024 + 1024 + 64 + 64 + 16 + 16 + 2048);
+    uint32_t* twl = h.data();
+    uint32_t* twli = twl + 1024;
+    uint32_t* tw64 = twli + 1024;
+    uint32_t* tw64i = tw64 + 64;
+    uint32_t* cr = tw64i + 64;
+    uint32_t* cri = cr + 16;
+    uint32_t* pw = cri + 16;
+    for (int k2 = 0; k2 < 16; k2++)
+        for (int l = 0; l < 64; l++) {
+            const long long e = static_cast<long long>(l) * (1 + 2 * k2);
+            twl[k2 * 64 + l] = host_to_mont(host_psi_pow(e));
+            twli[k2 * 64 + l] = host_to_mont(host_psi_pow(-e));
+        }
+    // omega = psi^2, omega^(16 a c) = psi^(32 a c)
+    for (int c = 0; c < 8; c++)
+        for (int a = 0; a < 8; a++) {
+            tw64[c * 8 + a] = host_to_mont(host_psi_pow(32LL * a * c));
+            tw64i[c * 8 + a] = host_to_mont(host_psi_pow(-32LL * a * c));
+        }
+    const uint32_t ninv = static_cast<uint32_t>(ntt31::powmod(N, ntt31::P - 2));
+    for (int r = 0; r < 16; r++) {
+        cr[r] = host_to_mont(host_psi_pow(64LL * r));
+        cri[r] = host_to_mont(host_mulmod(host_psi_pow(-64LL * r), ninv));
+    }
+    for (int t = 0; t < 2 * N; t++) pw[t] = host_to_mont(host_psi_pow(t));
+    CUFHE_HIP_TRY(hipMalloc(reinterpret_cast<void**>(&ts.mem), h.size() * sizeof(uint32_t)));
+    CUFHE_HIP_TRY(hipMemcpy(ts.mem, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    ts.tb.twl = ts.mem;
+    ts.tb.twli = ts.mem + (twli - twl);
+    ts.tb.tw64 = ts.mem + (tw64 - twl);
+    ts.tb.tw64i = ts.mem + (tw64i - twl);
+    ts.tb.cr = ts.mem + (cr - twl);
+    ts.tb.cri = ts.mem + (cri - twl);
+    ts.tb.pw = ts.mem + (pw - twl);
+    return CUFHE_OK;
+}
+
+void free_ntt_tables(NttTableSet& ts)
+{
+    (void)hipFree(ts.mem);
+    ts = NttTableSet{};
+}
+
 template <class P>
 FourierKeys<P> keys_view(const TableSet<P>& ts, const cplx* bk)
 {
@@ -136,6 +206,7 @@ template <class P>
 void free_keys(LevelKeys<P>& k)
 {
     (void)hipFree(k.bk);
+    (void)hipFree(k.bk_ntt);
     (void)hipFree(k.ksk);
     k = LevelKeys<P>{};
 }
@@ -151,6 +222,8 @@ int ensure_init()
     int rc = build_tables(s.t1);
     if (rc != CUFHE_OK) return rc;
     rc = build_tables(s.t2);
+    if (rc != CUFHE_OK) return rc;
+    rc = build_ntt_tables(s.tn);
     if (rc != CUFHE_OK) return rc;
     s.initialized = true;
     return CUFHE_OK;
@@ -194,6 +267,48 @@ int load_bk(LevelKeys<typename brP::targetP>& lk, const TableSet<typename brP::t
     (void)hipFree(tmp);
     if (e != hipSuccess) return fail(CUFHE_ERR_HIP, std::string("bk conversion: ") + hipGetErrorString(e));
     lk.kb = kb;
+    return CUFHE_OK;
+}
+
+// lvl1 key on the NTT backend: torus_to_ntt_mod + forward NTT per polynomial
+// (reference src/bootstrap_gpu.cu:212-264, :426-450), stored as u32
+// Montgomery residues in the transform's bin order.
+int load_bk_ntt(LevelKeys<lvl1param>& lk, const NttTableSet& ts, const uint32_t* bk,
+                size_t bk_words, int variant)
+{
+    using P = lvl1param;
+    const size_t ntrgsw = TFHEpp::kBKTRGSWs<TFHEpp::lvl01param>(variant == CUFHE_BR_KEY_BUNDLE);
+    const size_t want = ntrgsw * TFHEpp::kTRGSWWords<P>;
+    if (bk_words != want)
+        return fail(CUFHE_ERR_SIZE, "bootstrapping key has " + std::to_string(bk_words) +
+                                        " words, expected " + std::to_string(want));
+    const size_t npolys = ntrgsw * TFHEpp::BootstrappingTRGSWRows<P> * (P::k + 1);
+    void* tmp = nullptr;
+    CUFHE_HIP_TRY(hipMalloc(&tmp, want * sizeof(uint32_t)));
+    hipError_t e = hipMemcpy(tmp, bk, want * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(tmp);
+        return fail(CUFHE_ERR_HIP, std::string("bk upload: ") + hipGetErrorString(e));
+    }
+    e = hipMalloc(reinterpret_cast<void**>(&lk.bk_ntt), npolys * P::n * sizeof(uint32_t));
+    if (e != hipSuccess) {
+        (void)hipFree(tmp);
+        return fail(CUFHE_ERR_NOMEM, "bootstrapping key NTT allocation failed");
+    }
+    launch_to_ntt(lk.bk_ntt, static_cast<const uint32_t*>(tmp), static_cast<int>(npolys), ts.tb,
+                  nullptr);
+    e = hipDeviceSynchronize();
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(CUFHE_ERR_HIP, std::string("bk conversion: ") + hipGetErrorString(e));
+    lk.nk.bk = lk.bk_ntt;
+    lk.nk.tb = ts.tb;
+    // gadget identity h_q = 2^(32 - (q+1) Bgbit) as torus_to_ntt_mod, Montgomery form
+    for (int q = 0; q < static_cast<int>(P::l); q++) {
+        const uint64_t h = 1ULL << (32 - (q + 1) * P::Bgbit);
+        uint64_t v = (h * ntt31::P + (1ULL << 31)) >> 32;
+        if (v >= ntt31::P) v -= ntt31::P;
+        lk.nk.one_mont[q] = host_to_mont(static_cast<uint32_t>(v));
+    }
     return CUFHE_OK;
 }
 
@@ -241,6 +356,40 @@ int alloc_upload(T** dst, const std::vector<T>& v)
     return CUFHE_OK;
 }
 
+// Batched lvl1 blind rotation + SampleExtract on the loaded backend.
+void br_lvl1(const BrJob* jobs, int n, int c, uint32_t mu, hipStream_t st)
+{
+    const LevelKeys<lvl1param>& k = state().k1;
+    if (n <= 0) return;
+    if (k.backend == CUFHE_BACKEND_NTT)
+        launch_br_ntt(k.variant, jobs, n, c, mu, k.nk, st);
+    else if (k.kb)
+        launch_br_fft<lvl1param, true>(jobs, n, c, mu, k.fk, st);
+    else
+        launch_br_fft<lvl1param, false>(jobs, n, c, mu, k.fk, st);
+}
+
+// Batched lvl1 -> lvl0 identity key switch (full or subset key).
+void ks_lvl1(const KsJob* jobs, int n, hipStream_t st)
+{
+    const LevelKeys<lvl1param>& k = state().k1;
+    if (n > 0) launch_ks_batch(1, k.subset, jobs, n, k.ksk, st);
+}
+
+// One lvl1 gate (IKS-BR) as a single launch on the loaded backend.
+void gate_lvl1_one(bool mux, uint32_t* out, const uint32_t* in0, const uint32_t* in1,
+                   const uint32_t* inc, int32_t ca, int32_t cb, uint32_t offset, int32_t mux_sign,
+                   hipStream_t st)
+{
+    const LevelKeys<lvl1param>& k = state().k1;
+    if (k.backend == CUFHE_BACKEND_NTT)
+        launch_gate_lvl1_ntt(k.variant, k.subset, mux, out, in0, in1, inc, ca, cb, offset, mux_sign,
+                             k.ksk, k.nk, st);
+    else
+        launch_gate_lvl1_fft(k.kb, mux, out, in0, in1, inc, ca, cb, offset, mux_sign, k.ksk, k.fk,
+                             st);
+}
+
 }  // namespace
 }  // namespace cufhe_amd
 
@@ -280,15 +429,8 @@ int plan_launch(cufhe_plan* p, hipStream_t st)
         if (p->njobs_lin) launch_linear(p->lin, p->njobs_lin, st);
         return check_launch("batched lvl02 gate launch");
     }
-    if (p->njobs_ks) launch_ks_batch(1, false, p->ks, p->njobs_ks, s.k1.ksk, st);
-    if (p->njobs_br) {
-        if (s.k1.kb)
-            launch_br_fft<lvl1param, true>(p->br, p->njobs_br, p->c,
-                                           static_cast<uint32_t>(lvl1param::μ), s.k1.fk, st);
-        else
-            launch_br_fft<lvl1param, false>(p->br, p->njobs_br, p->c,
-                                            static_cast<uint32_t>(lvl1param::μ), s.k1.fk, st);
-    }
+    ks_lvl1(p->ks, p->njobs_ks, st);
+    br_lvl1(p->br, p->njobs_br, p->c, static_cast<uint32_t>(lvl1param::μ), st);
     if (p->njobs_lin) launch_linear(p->lin, p->njobs_lin, st);
     return check_launch("batched gate launch");
 }
@@ -328,18 +470,45 @@ int cufhe_initialize(void)
 int cufhe_initialize_lvl01(const uint32_t* bk, size_t bk_words, int key_bundle,
                            const uint32_t* ksk, size_t ksk_words)
 {
+    return cufhe_initialize_lvl01_ex(bk, bk_words, CUFHE_BACKEND_FFT,
+                                     key_bundle ? CUFHE_BR_KEY_BUNDLE : CUFHE_BR_PLAIN, ksk,
+                                     ksk_words, 0);
+}
+
+int cufhe_initialize_lvl01_ex(const uint32_t* bk, size_t bk_words, int backend, int variant,
+                              const uint32_t* ksk, size_t ksk_words, int subset_ksk)
+{
     if (!bk || !ksk) return fail(CUFHE_ERR_ARG, "null key pointer");
+    if (backend != CUFHE_BACKEND_FFT && backend != CUFHE_BACKEND_NTT)
+        return fail(CUFHE_ERR_ARG, "unknown backend " + std::to_string(backend));
+    if (variant < CUFHE_BR_PLAIN || variant > CUFHE_BR_BLOCK_BINARY)
+        return fail(CUFHE_ERR_ARG, "unknown blind-rotation variant " + std::to_string(variant));
+    if (backend == CUFHE_BACKEND_FFT && variant == CUFHE_BR_BLOCK_BINARY)
+        return fail(CUFHE_ERR_UNSUPPORTED, "block-binary blind rotation needs the NTT backend");
     State& s = state();
     std::lock_guard<std::mutex> g(s.mu);
     int rc = ensure_init();
     if (rc != CUFHE_OK) return rc;
+    (void)hipDeviceSynchronize();
     free_keys(s.k1);
-    rc = load_bk<TFHEpp::lvl01param>(s.k1, s.t1, bk, bk_words, key_bundle != 0);
-    if (rc == CUFHE_OK) rc = load_ksk(&s.k1.ksk, ksk, ksk_words, TFHEpp::kKSKWords<TFHEpp::lvl10param>);
+    if (backend == CUFHE_BACKEND_NTT)
+        rc = load_bk_ntt(s.k1, s.tn, bk, bk_words, variant);
+    else
+        rc = load_bk<TFHEpp::lvl01param>(s.k1, s.t1, bk, bk_words, variant == CUFHE_BR_KEY_BUNDLE);
+    using KS = TFHEpp::lvl10param;
+    const size_t ksk_want =
+        subset_ksk ? static_cast<size_t>(KS::domainP::n - KS::targetP::n) * KS::t *
+                         (1u << (KS::basebit - 1)) * (KS::targetP::n + 1)
+                   : TFHEpp::kKSKWords<KS>;
+    if (rc == CUFHE_OK) rc = load_ksk(&s.k1.ksk, ksk, ksk_words, ksk_want);
     if (rc != CUFHE_OK) {
         free_keys(s.k1);
         return rc;
     }
+    s.k1.backend = backend;
+    s.k1.variant = variant;
+    s.k1.kb = variant == CUFHE_BR_KEY_BUNDLE;
+    s.k1.subset = subset_ksk != 0;
     s.k1.loaded = true;
     return CUFHE_OK;
 }
@@ -373,6 +542,7 @@ int cufhe_cleanup(void)
     free_keys(s.k2);
     free_tables(s.t1);
     free_tables(s.t2);
+    free_ntt_tables(s.tn);
     s.initialized = false;
     return CUFHE_OK;
 }
@@ -489,15 +659,14 @@ int cufhe_gate_lvl1(int op, uint32_t* out, const uint32_t* in0, const uint32_t* 
     }
     if (op == CUFHE_MUX || op == CUFHE_NMUX) {
         if (!in1 || !inc) return fail(CUFHE_ERR_ARG, "null MUX operand");
-        launch_gate_lvl1_fft(s.k1.kb, true, out, in0, in1, inc, 0, 0, 0u,
-                             op == CUFHE_MUX ? 1 : -1, s.k1.ksk, s.k1.fk, st);
+        gate_lvl1_one(true, out, in0, in1, inc, 0, 0, 0u, op == CUFHE_MUX ? 1 : -1, st);
         return check_launch("mux gate");
     }
     GateLin gl;
     if (!gate_lin(op, gl)) return fail(CUFHE_ERR_ARG, "unknown gate op " + std::to_string(op));
     if (!in1) return fail(CUFHE_ERR_ARG, "null second operand");
-    launch_gate_lvl1_fft(s.k1.kb, false, out, in0, in1, nullptr, gl.ca, gl.cb,
-                         static_cast<uint32_t>(gl.off_mu) * mu, 0, s.k1.ksk, s.k1.fk, st);
+    gate_lvl1_one(false, out, in0, in1, nullptr, gl.ca, gl.cb,
+                  static_cast<uint32_t>(gl.off_mu) * mu, 0, st);
     return check_launch("gate");
 }
 
@@ -774,16 +943,9 @@ int cufhe_plan_run_timed(cufhe_plan* p, void* stream, float* kernel_ms)
     hipEvent_t ev[4];
     for (auto& e : ev) CUFHE_HIP_TRY(hipEventCreate(&e));
     CUFHE_HIP_TRY(hipEventRecord(ev[0], st));
-    if (p->njobs_ks) launch_ks_batch(1, false, p->ks, p->njobs_ks, s.k1.ksk, st);
+    ks_lvl1(p->ks, p->njobs_ks, st);
     CUFHE_HIP_TRY(hipEventRecord(ev[1], st));
-    if (p->njobs_br) {
-        if (s.k1.kb)
-            launch_br_fft<lvl1param, true>(p->br, p->njobs_br, p->c,
-                                           static_cast<uint32_t>(lvl1param::μ), s.k1.fk, st);
-        else
-            launch_br_fft<lvl1param, false>(p->br, p->njobs_br, p->c,
-                                            static_cast<uint32_t>(lvl1param::μ), s.k1.fk, st);
-    }
+    br_lvl1(p->br, p->njobs_br, p->c, static_cast<uint32_t>(lvl1param::μ), st);
     CUFHE_HIP_TRY(hipEventRecord(ev[2], st));
     if (p->njobs_lin) launch_linear(p->lin, p->njobs_lin, st);
     CUFHE_HIP_TRY(hipEventRecord(ev[3], st));
@@ -827,7 +989,7 @@ int cufhe_identity_key_switch_lvl10(uint32_t* d_out, const uint32_t* d_in, int n
     KsJob* d = nullptr;
     CUFHE_HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&d), n * sizeof(KsJob), st));
     CUFHE_HIP_TRY(hipMemcpyAsync(d, jobs.data(), n * sizeof(KsJob), hipMemcpyHostToDevice, st));
-    launch_ks_batch(1, false, d, n, s.k1.ksk, st);
+    ks_lvl1(d, n, st);
     CUFHE_HIP_TRY(hipFreeAsync(d, st));
     CUFHE_HIP_TRY(hipStreamSynchronize(st));
     return check_launch("identity key switch");
@@ -850,14 +1012,51 @@ int cufhe_bootstrap_lvl01(uint32_t* d_out, const uint32_t* d_in, uint32_t mu, in
     BrJob* d = nullptr;
     CUFHE_HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&d), n * sizeof(BrJob), st));
     CUFHE_HIP_TRY(hipMemcpyAsync(d, jobs.data(), n * sizeof(BrJob), hipMemcpyHostToDevice, st));
-    if (s.k1.kb)
-        launch_br_fft<lvl1param, true>(d, n, c, mu, s.k1.fk, st);
-    else
-        launch_br_fft<lvl1param, false>(d, n, c, mu, s.k1.fk, st);
+    br_lvl1(d, n, c, mu, st);
     CUFHE_HIP_TRY(hipFreeAsync(d, st));
     CUFHE_HIP_TRY(hipStreamSynchronize(st));
     return check_launch("bootstrap");
 }
+
+int cufhe_bootstrap_lvl02(uint64_t* d_out, const uint32_t* d_in, uint64_t mu, int n, void* stream)
+{
+    State& s = state();
+    if (!s.k2.loaded) return fail(CUFHE_ERR_STATE, "lvl02 keys not loaded");
+    if (n <= 0) return n == 0 ? CUFHE_OK : fail(CUFHE_ERR_ARG, "negative count");
+    if (!d_out || !d_in) return fail(CUFHE_ERR_ARG, "null pointer");
+    const hipStream_t st = as_stream(stream);
+    std::vector<BrJob> jobs(n);
+    for (int i = 0; i < n; i++)
+        jobs[i] = BrJob{d_in + static_cast<size_t>(i) * (lvl0param::n + 1), nullptr,
+                        d_out + static_cast<size_t>(i) * (lvl2param::n + 1), 1, 0, 0u, 0u};
+    BrJob* d = nullptr;
+    CUFHE_HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&d), n * sizeof(BrJob), st));
+    CUFHE_HIP_TRY(hipMemcpyAsync(d, jobs.data(), n * sizeof(BrJob), hipMemcpyHostToDevice, st));
+    if (s.k2.kb)
+        launch_br_fft<lvl2param, true>(d, n, 1, mu, s.k2.fk, st);
+    else
+        launch_br_fft<lvl2param, false>(d, n, 1, mu, s.k2.fk, st);
+    CUFHE_HIP_TRY(hipFreeAsync(d, st));
+    CUFHE_HIP_TRY(hipStreamSynchronize(st));
+    return check_launch("bootstrap lvl02");
+}
+
+int cufhe_ntt_transform(const uint32_t* d_in, uint32_t* d_out, int count, int inverse,
+                        void* stream)
+{
+    State& s = state();
+    {
+        std::lock_guard<std::mutex> g(s.mu);
+        const int rc = ensure_init();
+        if (rc != CUFHE_OK) return rc;
+    }
+    if (count < 0 || !d_in || !d_out) return fail(CUFHE_ERR_ARG, "bad arguments");
+    if (count == 0) return CUFHE_OK;
+    launch_ntt_natural(inverse != 0, d_out, d_in, count, s.tn.tb, as_stream(stream));
+    return check_launch("ntt transform");
+}
+
+uint32_t cufhe_ntt_root(void) { return ntt31::PSI; }
 
 int cufhe_ffnt_forward(int N, const void* d_poly, int integer, void* d_spectra, int count,
                        void* stream)

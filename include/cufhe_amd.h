@@ -81,6 +81,26 @@ int cufhe_initialize(void);
  * converted to the Fourier domain on the GPU. */
 int cufhe_initialize_lvl01(const uint32_t* bk, size_t bk_words, int key_bundle,
                            const uint32_t* ksk, size_t ksk_words);
+
+/* Backends and blind-rotation variants of the lvl1 keys (reference build
+ * options USE_FFT, USE_KEY_BUNDLE, USE_BLOCK_BINARY, USE_SUBSET_KEY in
+ * CMakeLists.txt:18-48). FFT: FP64 negacyclic FFNT; NTT: exact arithmetic
+ * mod P = 2147473409 (include/ntt_small_modulus.cuh:38-74). Block-binary
+ * needs the NTT backend and a block-binary lvl0 key (tfhe_secret_key_gen). */
+#define CUFHE_BACKEND_FFT 0
+#define CUFHE_BACKEND_NTT 1
+#define CUFHE_BR_PLAIN 0
+#define CUFHE_BR_KEY_BUNDLE 1
+#define CUFHE_BR_BLOCK_BINARY 2
+/* Initialize(ek) with an explicit backend and variant. bk: polynomial-domain
+ * BootstrappingKey<lvl01param> (key-bundle layout for CUFHE_BR_KEY_BUNDLE,
+ * one TRGSW per lvl0 key bit otherwise); ksk: KeySwitchingKey<lvl10param>,
+ * or the subset key (rows j >= n0 only, tfhe_ksk_subset_words() words) when
+ * subset_ksk != 0 (reference emplacesubiksk, test/test_block_binary_gpu.cc
+ * :153-156). cufhe_initialize_lvl01(bk, w, kb, ksk, w2) is
+ * cufhe_initialize_lvl01_ex(bk, w, CUFHE_BACKEND_FFT, kb, ksk, w2, 0). */
+int cufhe_initialize_lvl01_ex(const uint32_t* bk, size_t bk_words, int backend, int variant,
+                              const uint32_t* ksk, size_t ksk_words, int subset_ksk);
 /* Initialize_lvl02(ek, sk): BootstrappingKey<lvl02param> (uint64 words) and
  * KeySwitchingKey<lvl20param>. */
 int cufhe_initialize_lvl02(const uint64_t* bk, size_t bk_words, int key_bundle,
@@ -161,6 +181,11 @@ int cufhe_identity_key_switch_lvl10(uint32_t* d_out, const uint32_t* d_in, int n
  * in: n x 637, out: n x 1025 words. */
 int cufhe_bootstrap_lvl01(uint32_t* d_out, const uint32_t* d_in, uint32_t mu, int n,
                           int jobs_per_group, void* stream);
+/* The same at lvl2 with the lvl02 keys (reference src/bootstrap_gpu.cu
+ * :1248-1273, the rotation + SampleExtract half of the BR-IKS gate): in
+ * n x 637 uint32 lvl0 TLWEs, out n x 2049 uint64 lvl2 TLWEs. */
+int cufhe_bootstrap_lvl02(uint64_t* d_out, const uint32_t* d_in, uint64_t mu, int n,
+                          void* stream);
 
 /* FFNT of count real polynomials of degree N (1024 or 2048). integer != 0:
  * int32 coefficients used as is; else torus words (uint32 for N=1024,
@@ -170,6 +195,14 @@ int cufhe_ffnt_forward(int N, const void* d_poly, int integer, void* d_spectra, 
                        void* stream);
 /* Inverse FFNT incl. 1/(N/2): spectra -> count x N doubles. */
 int cufhe_ffnt_inverse(int N, const void* d_spectra, double* d_poly, int count, void* stream);
+/* Negacyclic NTT mod P = 2147473409 of count polynomials of N = 1024
+ * residues in [0, P): out[k] = sum_i in[i] psi^(i (2k+1)) mod P in natural
+ * bin order (psi: the backend's primitive 2048-th root, reported by
+ * cufhe_ntt_root()); inverse != 0 maps such spectra back (incl. N^-1)
+ * (reference include/ntt_small_modulus.cuh:394-509). */
+int cufhe_ntt_transform(const uint32_t* d_in, uint32_t* d_out, int count, int inverse,
+                        void* stream);
+uint32_t cufhe_ntt_root(void);
 /* TRGSW (polynomial domain, TFHEpp layout) -> spectra, count TRGSWs. */
 int cufhe_trgsw_to_fourier(int N, const void* d_trgsw, void* d_spectra, int count,
                            void* stream);

@@ -14,6 +14,10 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 EXACT, FFT, NTT = 0, 1, 2
+# blind-rotation variant passed as `kb`: plain CMUX, 2-bit key bundle,
+# block-binary key (blocks of 4); SUBSET (added to the variant) selects the
+# subset key switch in the lvl1 gate functions.
+BR_PLAIN, BR_KB, BR_BB, SUBSET = 0, 1, 2, 4
 P31 = 2147473409
 GOLDILOCKS = 0xFFFFFFFF00000001
 

@@ -480,16 +480,36 @@ std::vector<std::vector<u64>> key_mod(const typename P::T* C)
 // ---------------------------------------------------------------- blind rotation
 u32 mod_switch(u32 phase, u32 nbit) { return phase >> (32 - 1 - nbit); }
 
+// Blind-rotation variants: plain CMUX per key bit, 2-bit key bundle
+// (gatebootstrapping_gpu.cuh:1187-1332), block-binary key in blocks of
+// BB_ELL bits (gatebootstrapping_gpu.cuh:661-797, :801-884).
+enum Variant { BR_PLAIN = 0, BR_KB = 1, BR_BB = 2 };
+
+// Block-binary body: b - correction/2 rounded, correction = sum_i (a_i -
+// round(a_i)) as a wrapped signed 32-bit value (gatebootstrapping_gpu.cuh:814-838).
 template <class P>
-void blind_rotate(int mode, bool kb, const u32* tl0, const typename P::T* bk, typename P::T mu,
+u32 bb_corrected_body(const u32* tl0)
+{
+    constexpr u32 shift = 32 - 1 - P::nbit;
+    constexpr u32 rnd = 1u << (shift - 1);
+    u32 sum = 0;
+    for (u32 i = 0; i < N0; i++) sum += tl0[i] - ((mod_switch(tl0[i] + rnd, P::nbit)) << shift);
+    const i32 correction = static_cast<i32>(sum);
+    return tl0[N0] - static_cast<u32>(correction / 2) + rnd;
+}
+
+template <class P>
+void blind_rotate(int mode, int variant, const u32* tl0, const typename P::T* bk, typename P::T mu,
                   typename P::T* acc)
 {
     using T = typename P::T;
     constexpr u32 N = P::N, rows = 2 * P::l;
     constexpr size_t TW = static_cast<size_t>(rows) * 2 * N;
     const u32 rnd = 1u << (32 - 2 - P::nbit);
+    const bool kb = variant == BR_KB;
     // test vector (gatebootstrapping_gpu.cuh:31-53)
-    const u32 bar = 2 * N - mod_switch(tl0[N0], P::nbit);
+    const u32 body = variant == BR_BB ? bb_corrected_body<P>(tl0) : tl0[N0];
+    const u32 bar = 2 * N - mod_switch(body, P::nbit);
     for (u32 i = 0; i < N; i++) {
         acc[i] = 0;
         if (bar == 2 * N)
@@ -569,6 +589,45 @@ void blind_rotate(int mode, bool kb, const u32* tl0, const typename P::T* bk, ty
             acc_rot(b0, a01);
             extprod_exact<P>(acc, comb.data(), tmp.data());
             std::memcpy(acc, tmp.data(), 2 * N * sizeof(T));
+        }
+        return;
+    }
+    if (variant == BR_BB) {
+        // acc += ExtProd(acc, sum_e BK[ell b + e] (X^a_e - 1)), no identity term
+        for (u32 blk = 0; blk < N0 / BB_ELL; blk++) {
+            u32 a[BB_ELL];
+            for (u32 e = 0; e < BB_ELL; e++) a[e] = mod_switch(tl0[blk * BB_ELL + e] + rnd, P::nbit);
+            if (mode == ORACLE_NTT) {
+                const u64 Pm = ntt_prime<P>();
+                const Field f{Pm};
+                std::vector<std::vector<u64>> cm(rows * 2, std::vector<u64>(N, 0));
+                for (u32 e = 0; e < BB_ELL; e++) {
+                    const auto m = key_mod<P>(bk + (blk * BB_ELL + e) * TW);
+                    for (u32 p = 0; p < rows * 2; p++)
+                        for (u32 i = 0; i < N; i++) {
+                            const u32 sidx = (i - a[e]) & (N - 1);
+                            const bool neg = (i < (a[e] & (N - 1))) ^ ((a[e] / N) != 0);
+                            const u64 v = m[p][sidx] % Pm;
+                            u64& d = cm[p][i];
+                            d = neg ? f.sub(d, v) : f.add(d, v);
+                            d = f.sub(d, m[p][i] % Pm);
+                        }
+                }
+                for (u32 i = 0; i < 2 * N; i++) tmp[i] = 0;
+                extprod_ntt<P>(acc, cm, Pm, tmp.data(), true);
+            }
+            else {
+                for (size_t i = 0; i < TW; i++) comb[i] = 0;
+                for (u32 e = 0; e < BB_ELL; e++) {
+                    const T* src = bk + (blk * BB_ELL + e) * TW;
+                    for (u32 p = 0; p < rows * 2; p++) {
+                        rotate<T>(src + p * N, N, a[e], rot.data());
+                        for (u32 i = 0; i < N; i++) comb[p * N + i] += rot[i] - src[p * N + i];
+                    }
+                }
+                extprod_exact<P>(acc, comb.data(), tmp.data());
+            }
+            for (u32 i = 0; i < 2 * N; i++) acc[i] += tmp[i];
         }
         return;
     }
@@ -817,15 +876,17 @@ int oracle_external_product(int level, int mode, const void* d, const void* trgs
     return 0;
 }
 
-// Blind rotation of a lvl0 TLWE into a TRLWE of the target level.
+// Blind rotation of a lvl0 TLWE into a TRLWE of the target level; kb is the
+// variant (0 plain, 1 key bundle, 2 block-binary: exact and NTT modes only).
 int oracle_blind_rotate(int level, int mode, int kb, const u32* tl0, const void* bk, u64 mu,
                         void* trlwe_out)
 {
+    if (kb < 0 || kb > 2 || (kb == BR_BB && mode == ORACLE_FFT)) return -1;
     if (level == 1)
-        blind_rotate<L1>(mode, kb != 0, tl0, static_cast<const u32*>(bk), static_cast<u32>(mu),
+        blind_rotate<L1>(mode, kb, tl0, static_cast<const u32*>(bk), static_cast<u32>(mu),
                          static_cast<u32*>(trlwe_out));
     else if (level == 2)
-        blind_rotate<L2>(mode, kb != 0, tl0, static_cast<const u64*>(bk), mu,
+        blind_rotate<L2>(mode, kb, tl0, static_cast<const u64*>(bk), mu,
                          static_cast<u64*>(trlwe_out));
     else
         return -1;
@@ -859,10 +920,13 @@ int oracle_identity_key_switch(int level, int subset, const void* in0, const voi
     return 0;
 }
 
-// lvl1 gate API (IKS-BR order): lvl1 TLWEs in and out.
+// lvl1 gate API (IKS-BR order): lvl1 TLWEs in and out. kb: blind-rotation
+// variant (0 plain, 1 key bundle, 2 block-binary) plus 4 for a subset KSK.
 int oracle_gate_lvl1(int mode, int kb, int op, const u32* in0, const u32* in1, const u32* inc,
                      const u32* bk, const u32* ksk, u32* out)
 {
+    if ((kb & 3) == BR_BB && mode == ORACLE_FFT) return -1;
+    const bool subset = (kb & 4) != 0;
     constexpr u32 N = L1::N, mu = L1::mu;
     std::vector<u32> tl0(N0 + 1), acc(2 * N), r1(N + 1), r0(N + 1);
     if (op == 12 || op == 13) {  // NOT / COPY
@@ -870,11 +934,11 @@ int oracle_gate_lvl1(int mode, int kb, int op, const u32* in0, const u32* in1, c
         return 0;
     }
     if (op == 10 || op == 11) {  // MUX / NMUX
-        identity_key_switch<u32>(N, false, inc, in1, 1, 1, 0u - mu, ksk, tl0.data());
-        blind_rotate<L1>(mode, kb != 0, tl0.data(), bk, mu, acc.data());
+        identity_key_switch<u32>(N, subset, inc, in1, 1, 1, 0u - mu, ksk, tl0.data());
+        blind_rotate<L1>(mode, kb & 3, tl0.data(), bk, mu, acc.data());
         sample_extract<u32>(acc.data(), N, r1.data());
-        identity_key_switch<u32>(N, false, inc, in0, -1, 1, 0u - mu, ksk, tl0.data());
-        blind_rotate<L1>(mode, kb != 0, tl0.data(), bk, mu, acc.data());
+        identity_key_switch<u32>(N, subset, inc, in0, -1, 1, 0u - mu, ksk, tl0.data());
+        blind_rotate<L1>(mode, kb & 3, tl0.data(), bk, mu, acc.data());
         sample_extract<u32>(acc.data(), N, r0.data());
         for (u32 i = 0; i <= N; i++) {
             u32 v = r1[i] + r0[i];
@@ -885,9 +949,9 @@ int oracle_gate_lvl1(int mode, int kb, int op, const u32* in0, const u32* in1, c
     }
     GateLin g;
     if (!gate_lin(op, g)) return -1;
-    identity_key_switch<u32>(N, false, in0, in1, g.ca, g.cb, static_cast<u32>(g.off) * mu, ksk,
+    identity_key_switch<u32>(N, subset, in0, in1, g.ca, g.cb, static_cast<u32>(g.off) * mu, ksk,
                              tl0.data());
-    blind_rotate<L1>(mode, kb != 0, tl0.data(), bk, mu, acc.data());
+    blind_rotate<L1>(mode, kb & 3, tl0.data(), bk, mu, acc.data());
     sample_extract<u32>(acc.data(), N, out);
     return 0;
 }
@@ -914,10 +978,10 @@ int oracle_gate_lvl02(int mode, int kb, int op, const u32* in0, const u32* in1, 
     };
     if (op == 10 || op == 11) {
         pre(inc, in1, 1, 1, 0u - mu0);
-        blind_rotate<L2>(mode, kb != 0, tl0.data(), bk, mu, acc.data());
+        blind_rotate<L2>(mode, kb & 3, tl0.data(), bk, mu, acc.data());
         sample_extract<u64>(acc.data(), N, r1.data());
         pre(inc, in0, -1, 1, 0u - mu0);
-        blind_rotate<L2>(mode, kb != 0, tl0.data(), bk, mu, acc.data());
+        blind_rotate<L2>(mode, kb & 3, tl0.data(), bk, mu, acc.data());
         sample_extract<u64>(acc.data(), N, r0.data());
         const i32 s = op == 10 ? 1 : -1;
         identity_key_switch<u64>(N, false, r1.data(), r0.data(), s, s, s > 0 ? mu : 0 - mu, ksk,
@@ -927,7 +991,7 @@ int oracle_gate_lvl02(int mode, int kb, int op, const u32* in0, const u32* in1, 
     GateLin g;
     if (!gate_lin(op, g)) return -1;
     pre(in0, in1, g.ca, g.cb, static_cast<u32>(g.off) * mu0);
-    blind_rotate<L2>(mode, kb != 0, tl0.data(), bk, mu, acc.data());
+    blind_rotate<L2>(mode, kb & 3, tl0.data(), bk, mu, acc.data());
     sample_extract<u64>(acc.data(), N, r1.data());
     identity_key_switch<u64>(N, false, r1.data(), nullptr, 1, 0, 0, ksk, out);
     return 0;

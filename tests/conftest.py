@@ -39,6 +39,32 @@ def keys_lvl2(keys_lvl1):
     }
 
 
+@pytest.fixture(scope="session")
+def keys_bb():
+    """Block-binary lvl0 key (blocks of 4, at most one bit each) whose lvl1 key
+    starts with it; plain lvl01 BK (one TRGSW per bit) and the subset KSK."""
+    import cufhe_amd as C
+    k0, k1, k2 = C.secret_key_gen(3, block_binary=True)
+    return {
+        "k0": k0, "k1": k1,
+        "bk": C.bk_gen(1, False, 3, k0, k1),
+        "ksk_subset": C.ksk_gen(1, 3, k1, k0, subset=True),
+    }
+
+
+_LOADED_LVL1 = {"tag": None}
+
+
+def load_lvl1_keys(tag, bk, ksk, backend=0, variant=1, subset=False):
+    """Load lvl1 gate keys on the GPU unless the same set (tag) is loaded."""
+    import cufhe_amd as C
+    if _LOADED_LVL1["tag"] == tag:
+        return
+    _LOADED_LVL1["tag"] = None
+    C.initialize_lvl01(bk, ksk, backend=backend, variant=variant, subset=subset)
+    _LOADED_LVL1["tag"] = tag
+
+
 def signed32(x):
     return np.asarray(x, np.uint32).view(np.int32).astype(np.int64)
 

@@ -12,7 +12,7 @@ import pytest
 
 import cufhe_amd as C
 import oracle as O
-from conftest import signed32
+from conftest import load_lvl1_keys, signed32
 
 pytestmark = pytest.mark.gpu
 
@@ -39,17 +39,9 @@ def engine():
     yield C.lib()
 
 
-_loaded = {}
-
-
 def load_lvl1(keys, kb):
-    if _loaded.get("lvl1") == kb:
-        return
-    bk = keys["bk_kb"] if kb else keys["bk"]
-    ksk = keys["ksk"]
-    C.check(C.lib().cufhe_initialize_lvl01(C._ptr(bk), bk.size, int(kb), C._ptr(ksk), ksk.size),
-            "cufhe_initialize_lvl01")
-    _loaded["lvl1"] = kb
+    load_lvl1_keys(("fft", kb), keys["bk_kb"] if kb else keys["bk"], keys["ksk"],
+                   backend=C.BACKEND_FFT, variant=C.BR_KEY_BUNDLE if kb else C.BR_PLAIN)
 
 
 def gpu_forward(n, polys, integer=True):

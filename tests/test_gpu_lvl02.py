@@ -81,11 +81,40 @@ def test_lvl02_plain_and_batched(lvl02):
         batched = dpool2.download(np.uint32, pool.shape)[3 * n:]
         plan.destroy()
         assert np.array_equal(batched, plain), graph
-    # oracle parity on two gates (exact-integer lvl2 external products)
+    # Oracle parity of whole gates (exact-integer lvl2 external products).
+    # FP64 cannot hold Torus64 exactly: once one gadget digit of a rotation
+    # differs the masks diverge, and the lvl2 -> lvl0 key switch then draws
+    # different KSK rows, so the outputs are different lvl0 samples of the
+    # same message. The bar is the message and a phase difference within the
+    # key-switch noise (sigma ~ 0.01 for 2 x 2048 x 7 lvl0-noise rows).
     for i in [0, 42]:
         e = O.gate_lvl02(O.EXACT, True, C.GATES[ops[i]], pool[i], pool[n + i], pool[2 * n + i],
                          lvl02["bk_kb"], lvl02["ksk"])
-        eq = np.mean(e == plain[i])
         dph = abs(C.phase(0, k0, e[None, :])[0] - C.phase(0, k0, plain[i][None, :])[0])
-        print(ops[i], "lvl0 coefficient equality", eq, "phase diff", dph)
-        assert eq >= 0.99 and dph < 2 ** -20
+        print(ops[i], "lvl0 phase diff vs oracle", dph)
+        assert C.decrypt_bits(0, k0, e[None, :])[0] == want[i]
+        assert dph < 0.08
+
+
+def test_lvl02_rotation_phase_matches_exact_oracle(lvl02):
+    """lvl2 rotation + SampleExtract against the exact-integer oracle before
+    the key switch: both are Torus64 samples of the same test-vector
+    coefficient; their phases differ by the rotation noise (sigma ~ 2^-24),
+    far below the 2^-20 tolerance, while a wrong rotation index, sign or
+    key-bundle term moves the phase by mu = 1/8."""
+    k0, k2 = lvl02["k0"], lvl02["k2"]
+    rng = np.random.default_rng(202)
+    n = 4
+    tl0 = rng.integers(0, 2**32, (n, W0), dtype=np.uint64).astype(np.uint32)
+    din = C.DeviceBuffer.from_numpy(tl0)
+    dout = C.DeviceBuffer(n * 2049 * 8)
+    mu2 = 1 << 61
+    C.check(C.lib().cufhe_bootstrap_lvl02(dout.ptr, din.ptr, mu2, n, None), "bootstrap_lvl02")
+    g = dout.download(np.uint64, (n, 2049))
+    for i in range(n):
+        e = O.sample_extract(2, O.blind_rotate(2, O.EXACT, True, tl0[i], lvl02["bk_kb"], mu2))
+        pg = C.phase(2, k2, g[i][None, :])[0]
+        pe = C.phase(2, k2, e[None, :])[0]
+        print("lvl2 phase gpu", pg, "exact", pe)
+        assert abs(abs(pe) - 0.125) < 2 ** -16
+        assert abs(pg - pe) < 2 ** -20
