@@ -1,16 +1,23 @@
 #!/usr/bin/env python3
-"""Benchmark of the batched lvl1 gate-bootstrapping path on one MI355X.
+"""Benchmarks of the gate-bootstrapping path on one MI355X.
 
-Workload (the batched key-bundle configuration): 4096 independent lvl1
-binary gates, gate type uniform over {NAND, AND, OR, XOR, XNOR, NOR}, random
-input bits from a seed, key bundle KB=2, FFT backend, n=636 / N=1024 / l=2 /
-Bg=2^8. One step = one pass of the hot path over the batch with inputs
-resident in HBM: identity key switching of all gates, then the fused
-blind-rotation + sample-extraction kernel. Keys are generated from seed 0.
+Default workload (the batched key-bundle configuration, the headline line):
+4096 independent lvl1 binary gates, gate type uniform over {NAND, AND, OR,
+XOR, XNOR, NOR}, random input bits from a seed, key bundle KB=2, FFT backend,
+n=636 / N=1024 / l=2 / Bg=2^8. One step = one pass of the hot path over the
+batch with inputs resident in HBM: identity key switching of all gates, then
+the fused blind-rotation + sample-extraction kernel. Keys from seed 0.
 
-Prints ONE JSON line (rank 0). The plain per-gate form (one gate per launch,
-round-robin over streams, the reference's form) is measured on a sample and
-reported beside the batched figure, with the CPU oracle as the CPU baseline.
+Other workloads (--workload), each printing one JSON line of the same shape:
+  lvl2mix  batched lvl02 gates (N=2048, Torus64, KB=2, FFT): 1536 binary +
+           512 MUX (2 bootstraps each), per-gate-type throughput vs the plain
+           per-gate form;
+  nttbb    NTT backend, block-binary lvl0 key, subset key switching: 4096
+           NAND/XOR gates, bit-exact against the oracle on a sample.
+
+The plain per-gate form (one gate per launch, round-robin over streams, the
+reference's form) is measured on a sample beside each batched figure; the CPU
+oracle on the host cores is the CPU baseline of the default workload.
 """
 import argparse
 import json
@@ -24,7 +31,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "lvl1 NAND bootstraps/s @batch 4096 vs plain per-gate form; p50/p99 layer latency"
 FLOP_PER_BOOTSTRAP_KB = 91502592      # 318 x 287,744 (algorithmic, lvl1 KB=2)
+FLOP_PER_BOOTSTRAP_KB_LVL2 = 349077504  # 318 x 1,097,728 (lvl2 KB=2)
 BK_BYTES_KB = 62521344                # 318 x 3 TRGSW x 8 polys x 512 bins x 16 B
+BK_BYTES_NTT_BB = 20840448            # 636 TRGSW x 8 polys x 1024 bins x 4 B
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBPS = 8000.0
 GATE_MIX = ["NAND", "AND", "OR", "XOR", "XNOR", "NOR"]
@@ -42,27 +51,100 @@ def pct(v, q):
     return v[i]
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--batch", type=int, default=4096)
-    ap.add_argument("--jobs-per-group", type=int, default=0,
-                    help="bootstraps per workgroup; 0 = autotune over {1, 2, 4, 8}")
-    ap.add_argument("--plain-sample", type=int, default=256)
-    ap.add_argument("--plain-streams", type=int, default=64)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--seed", type=int, default=1)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py runs on one GPU: the gate batch is one GPU's share; "
-              "multi-GPU sharding (independent gates, replicated keys) is not part of this "
-              "build. Use --gpus 1.", file=sys.stderr)
-        sys.exit(2)
+def want_bit(op, x, y, s=0):
+    import cufhe_amd as C
+    if op == "MUX":
+        return y if s else x
+    if op == "NMUX":
+        return 1 - (y if s else x)
+    return C.TRUTH[op](x, y)
 
+
+def timed_steps(C, plan, steps):
+    """K eager steps with per-kernel HIP events; device synchronised on both
+    sides. Returns (total s, per-step ms, ks ms, br ms, combine ms)."""
+    C.check(C.lib().cufhe_synchronize(), "synchronize")
+    step_ms, ks_ms, br_ms, lin_ms = [], [], [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        k, b, l = plan.run_timed()
+        step_ms.append((time.perf_counter() - s0) * 1e3)
+        ks_ms.append(k)
+        br_ms.append(b)
+        lin_ms.append(l)
+    C.check(C.lib().cufhe_synchronize(), "synchronize")
+    return time.perf_counter() - t0, step_ms, ks_ms, br_ms, lin_ms
+
+
+def graph_ms(C, gates, dpool, c, level=1, reps=3):
+    gplan = C.Plan(gates, dpool, jobs_per_group=c, use_graph=True, level=level)
+    gplan.run()
+    C.check(C.lib().cufhe_synchronize(), "synchronize")
+    g0 = time.perf_counter()
+    for _ in range(reps):
+        gplan.run()
+    C.check(C.lib().cufhe_synchronize(), "synchronize")
+    ms = (time.perf_counter() - g0) * 1e3 / reps
+    gplan.destroy()
+    return ms
+
+
+def plain_form(C, gate_fn, calls, nstreams, lat_calls=5):
+    """Throughput of one-gate-per-launch calls round-robin over streams, and
+    single-gate latency on one stream."""
+    streams = [C.Stream() for _ in range(nstreams)]
+    for i, c in enumerate(calls[:8]):  # warm
+        gate_fn(*c, streams[i % nstreams].handle)
+    C.check(C.lib().cufhe_synchronize(), "synchronize")
+    p0 = time.perf_counter()
+    for i, c in enumerate(calls):
+        gate_fn(*c, streams[i % nstreams].handle)
+    C.check(C.lib().cufhe_synchronize(), "synchronize")
+    tput = len(calls) / (time.perf_counter() - p0)
+    lat = []
+    for c in calls[:lat_calls]:
+        l0 = time.perf_counter()
+        gate_fn(*c, streams[0].handle)
+        streams[0].synchronize()
+        lat.append((time.perf_counter() - l0) * 1e3)
+    for s in streams:
+        s.destroy()
+    return tput, pct(lat, 50)
+
+
+def autotune(C, gates, dpool, cands):
+    tune = {}
+    for c in cands:
+        plan = C.Plan(gates, dpool, jobs_per_group=c)
+        plan.run_timed()
+        tune[c] = min(plan.run_timed()[1] for _ in range(2))
+        log("autotune C=%d: blind rotation %.2f ms" % (c, tune[c]))
+        plan.destroy()
+    return min(tune, key=tune.get), tune
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch of the workload's blind-rotation kernel, from the
+    rocprofv3 FETCH_SIZE/WRITE_SIZE runs summarised under profiles/."""
+    tf = os.path.join(ROOT, "profiles", "br_kernel_traffic.json")
+    if not os.path.exists(tf):
+        return None
+    with open(tf) as f:
+        return json.load(f).get(workload, {}).get("hbm_bytes_per_launch")
+
+
+def base_result(args, value, ms_per_step, dtype, config):
+    return {
+        "metric": METRIC, "value": round(value, 2), "unit": "gates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+        "data": "synthetic", "config": config,
+    }
+
+
+# ------------------------------------------------------------------ default
+def workload_kb4096(args):
     import numpy as np
 
     import cufhe_amd as C
@@ -73,8 +155,7 @@ def main():
     bk = C.bk_gen(1, True, 0, k0, k1)
     ksk = C.ksk_gen(1, 0, k1, k0)
     C.check(C.lib().cufhe_initialize(), "initialize")
-    C.check(C.lib().cufhe_initialize_lvl01(C._ptr(bk), bk.size, 1, C._ptr(ksk), ksk.size),
-            "initialize_lvl01")
+    C.initialize_lvl01(bk, ksk, backend=C.BACKEND_FFT, variant=C.BR_KEY_BUNDLE)
 
     rng = np.random.default_rng(args.seed)
     ops = [GATE_MIX[i] for i in rng.integers(0, len(GATE_MIX), B)]
@@ -87,17 +168,8 @@ def main():
     want = np.array([C.TRUTH[o](int(bits[i]), int(bits[B + i])) for i, o in enumerate(ops)],
                     np.uint8)
 
-    # ---- autotune jobs per workgroup
     cands = [args.jobs_per_group] if args.jobs_per_group else [1, 2, 4, 8]
-    tune = {}
-    for c in cands:
-        plan = C.Plan(gates, dpool, jobs_per_group=c)
-        plan.run_timed()
-        t = [plan.run_timed()[1] for _ in range(2)]
-        tune[c] = min(t)
-        log("autotune C=%d: blind rotation %.2f ms" % (c, tune[c]))
-        plan.destroy()
-    best_c = min(tune, key=tune.get)
+    best_c, tune = autotune(C, gates, dpool, cands)
 
     plan = C.Plan(gates, dpool, jobs_per_group=best_c)
     nbr = plan.bootstraps
@@ -105,79 +177,32 @@ def main():
     for _ in range(args.warmup):
         plan.run_timed()
     out = dpool.download(np.uint32, (3 * B, C.TLWE1_WORDS))[2 * B:]
-    dec = C.decrypt_bits(1, k1, out)
-    failures = int(np.sum(dec != want))
+    failures = int(np.sum(C.decrypt_bits(1, k1, out) != want))
     log("decryption failures after warmup: %d / %d" % (failures, B))
 
-    # ---- timed region: exactly K steps, device synchronised on both sides
-    C.check(C.lib().cufhe_synchronize(), "synchronize")
-    step_ms, ks_ms, br_ms = [], [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s0 = time.perf_counter()
-        k, b, _ = plan.run_timed()
-        step_ms.append((time.perf_counter() - s0) * 1e3)
-        ks_ms.append(k)
-        br_ms.append(b)
-    C.check(C.lib().cufhe_synchronize(), "synchronize")
-    total = time.perf_counter() - t0
+    total, step_ms, ks_ms, br_ms, _ = timed_steps(C, plan, args.steps)
     value = B * args.steps / total
     ms_per_step = total * 1e3 / args.steps
     log("batched: %.1f gates/s, %.2f ms/step (ks %.2f ms, br %.2f ms)" %
         (value, ms_per_step, statistics.mean(ks_ms), statistics.mean(br_ms)))
+    gms = graph_ms(C, gates, dpool, best_c)
 
-    # graph replay of the same plan (launch overhead removed)
-    gplan = C.Plan(gates, dpool, jobs_per_group=best_c, use_graph=True)
-    gplan.run()
-    C.check(C.lib().cufhe_synchronize(), "synchronize")
-    g0 = time.perf_counter()
-    for _ in range(3):
-        gplan.run()
-    C.check(C.lib().cufhe_synchronize(), "synchronize")
-    graph_ms = (time.perf_counter() - g0) * 1e3 / 3
-    gplan.destroy()
-
-    # ---- plain per-gate form on a sample (one gate per launch, many streams)
     P = min(args.plain_sample, B)
-    streams = [C.Stream() for _ in range(args.plain_streams)]
     W = C.TLWE1_WORDS * 4
-    for i in range(min(8, P)):  # warm
-        C.check(C.lib().cufhe_gate_lvl1(C.GATES[ops[i]], dpool.at((2 * B + i) * W),
-                                        dpool.at(i * W), dpool.at((B + i) * W), None,
-                                        streams[i % len(streams)].handle), "gate")
-    C.check(C.lib().cufhe_synchronize(), "synchronize")
-    p0 = time.perf_counter()
-    for i in range(P):
-        C.check(C.lib().cufhe_gate_lvl1(C.GATES[ops[i]], dpool.at((2 * B + i) * W),
-                                        dpool.at(i * W), dpool.at((B + i) * W), None,
-                                        streams[i % len(streams)].handle), "gate")
-    C.check(C.lib().cufhe_synchronize(), "synchronize")
-    plain_tput = P / (time.perf_counter() - p0)
-    lat = []
-    for i in range(5):
-        l0 = time.perf_counter()
-        C.check(C.lib().cufhe_gate_lvl1(C.GATES[ops[i]], dpool.at((2 * B + i) * W),
-                                        dpool.at(i * W), dpool.at((B + i) * W), None,
-                                        streams[0].handle), "gate")
-        streams[0].synchronize()
-        lat.append((time.perf_counter() - l0) * 1e3)
-    for s in streams:
-        s.destroy()
-    log("plain form: %.1f gates/s over %d streams; single-gate latency p50 %.2f ms" %
-        (plain_tput, len(streams), pct(lat, 50)))
 
-    # ---- roofline of the dominant kernel (blind rotation)
+    def gate_fn(i, st):
+        C.check(C.lib().cufhe_gate_lvl1(C.GATES[ops[i]], dpool.at((2 * B + i) * W),
+                                        dpool.at(i * W), dpool.at((B + i) * W), None, st), "gate")
+    plain_tput, plain_lat = plain_form(C, gate_fn, [(i,) for i in range(P)], args.plain_streams)
+    log("plain form: %.1f gates/s over %d streams; single-gate latency p50 %.2f ms" %
+        (plain_tput, args.plain_streams, plain_lat))
+
     br_avg_s = statistics.mean(br_ms) / 1e3
     achieved = FLOP_PER_BOOTSTRAP_KB * nbr / br_avg_s / 1e12
     groups = -(-nbr // best_c)
     key_gbps = BK_BYTES_KB * groups / br_avg_s / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "br_kernel_traffic.json")
-    if os.path.exists(tf):
-        with open(tf) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    traffic = measured_traffic("kb4096")
 
-    # ---- CPU baseline: the oracle (FP64 FFNT mirror + integer KS) on host cores
     cpu = None
     if not args.no_cpu:
         import oracle as O
@@ -196,33 +221,21 @@ def main():
                          "key switch), one gate per thread; not TFHEpp" % done}
         log("cpu baseline: %.2f gates/s on %d threads" % (cpu["value"], nthr))
 
-    res = {
-        "metric": METRIC,
-        "value": round(value, 2),
-        "unit": "gates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {
-            "workload": "batched lvl1 binary gates, KB=2, FFT, 4096 gates uniform over "
-                        "NAND/AND/OR/XOR/XNOR/NOR, inputs resident in HBM",
-            "global_batch": B, "n": 636, "N": 1024, "l": 2, "Bgbit": 8,
-            "jobs_per_group": best_c, "autotune_br_ms": {str(k): round(v, 3) for k, v in tune.items()},
-            "parallelism": "single GPU",
-        },
+    res = base_result(args, value, ms_per_step, "f64", {
+        "workload": "batched lvl1 binary gates, KB=2, FFT, 4096 gates uniform over "
+                    "NAND/AND/OR/XOR/XNOR/NOR, inputs resident in HBM",
+        "global_batch": B, "n": 636, "N": 1024, "l": 2, "Bgbit": 8,
+        "jobs_per_group": best_c, "autotune_br_ms": {str(k): round(v, 3) for k, v in tune.items()},
+        "parallelism": "single GPU",
+    })
+    res.update({
         "p50_layer_ms": round(pct(step_ms, 50), 3),
         "p99_layer_ms": round(pct(step_ms, 99), 3),
-        "graph_ms_per_step": round(graph_ms, 3),
+        "graph_ms_per_step": round(gms, 3),
         "kernel_ms": {"key_switch": round(statistics.mean(ks_ms), 3),
                       "blind_rotate": round(statistics.mean(br_ms), 3)},
         "plain_form": {"gates_per_s": round(plain_tput, 2), "streams": args.plain_streams,
-                       "sample": P, "latency_p50_ms": round(pct(lat, 50), 3)},
+                       "sample": P, "latency_p50_ms": round(plain_lat, 3)},
         "batched_over_plain": round(value / plain_tput, 2),
         "decrypt_failures": failures,
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
@@ -230,7 +243,211 @@ def main():
                      "traffic": traffic},
         "key_stream": {"algorithmic_GBps": round(key_gbps, 1), "peak_GBps": HBM_PEAK_GBPS},
         "cpu_baseline": cpu,
-    }
+    })
+    return res
+
+
+# ------------------------------------------------------------------ lvl2 mix
+def workload_lvl2mix(args):
+    import numpy as np
+
+    import cufhe_amd as C
+
+    nbin, nmux = 1536, 512
+    log("keys (seed 0, lvl02 key bundle)")
+    k0, _, k2 = C.secret_key_gen(0)
+    bk = C.bk_gen(2, True, 0, k0, k2)
+    ksk = C.ksk_gen(2, 0, k2, k0)
+    C.check(C.lib().cufhe_initialize(), "initialize")
+    C.check(C.lib().cufhe_initialize_lvl02(C._ptr(bk), bk.size, 1, C._ptr(ksk), ksk.size),
+            "initialize_lvl02")
+    rng = np.random.default_rng(args.seed)
+    ops = [GATE_MIX[i] for i in rng.integers(0, len(GATE_MIX), nbin)] + ["MUX"] * nmux
+    perm = rng.permutation(len(ops))
+    ops = [ops[i] for i in perm]  # uneven mix, interleaved
+    n = len(ops)
+    bits = rng.integers(0, 2, 3 * n).astype(np.uint8)
+    W0 = C.TLWE0_WORDS
+    pool = np.zeros((4 * n, W0), np.uint32)
+    pool[: 3 * n] = C.encrypt_bits(0, 2000 + args.seed, k0, bits)
+    want = np.array([want_bit(o, int(bits[i]), int(bits[n + i]), int(bits[2 * n + i]))
+                     for i, o in enumerate(ops)], np.uint8)
+    dpool = C.DeviceBuffer.from_numpy(pool)
+
+    def run_subset(idx, label):
+        gates = [(ops[i], 3 * n + i, i, n + i, 2 * n + i) for i in idx]
+        plan = C.Plan(gates, dpool, level=2)
+        for _ in range(args.warmup):
+            plan.run_timed()
+        total, step_ms, ks_ms, br_ms, lin_ms = timed_steps(C, plan, args.steps)
+        nb = plan.bootstraps
+        plan.destroy()
+        r = {"gates": len(idx), "bootstraps": nb,
+             "gates_per_s": round(len(idx) * args.steps / total, 2),
+             "bootstraps_per_s": round(nb * args.steps / total, 2),
+             "ms_per_step": round(total * 1e3 / args.steps, 3),
+             "p50_layer_ms": round(pct(step_ms, 50), 3), "p99_layer_ms": round(pct(step_ms, 99), 3),
+             "kernel_ms": {"blind_rotate": round(statistics.mean(br_ms), 3),
+                           "key_switch": round(statistics.mean(ks_ms), 3)}}
+        log("lvl2 %s: %s" % (label, r))
+        return r, statistics.mean(br_ms), nb
+
+    mix, br_mix_ms, nb_mix = run_subset(list(range(n)), "mix")
+    out = dpool.download(np.uint32, pool.shape)[3 * n:]
+    failures = int(np.sum(C.decrypt_bits(0, k0, out) != want))
+    binr, _, _ = run_subset([i for i in range(n) if ops[i] != "MUX"], "binary")
+    muxr, _, _ = run_subset([i for i in range(n) if ops[i] == "MUX"], "mux")
+
+    Wb = W0 * 4
+
+    def gate_fn(i, st):
+        C.check(C.lib().cufhe_gate_lvl02(C.GATES[ops[i]], dpool.at((3 * n + i) * Wb),
+                                         dpool.at(i * Wb), dpool.at((n + i) * Wb),
+                                         dpool.at((2 * n + i) * Wb), st), "gate_lvl02")
+    sample = max(16, args.plain_sample // 4)
+    bin_idx = [i for i in range(n) if ops[i] != "MUX"][:sample]
+    mux_idx = [i for i in range(n) if ops[i] == "MUX"][:sample]
+    pb, pb_lat = plain_form(C, gate_fn, [(i,) for i in bin_idx], args.plain_streams)
+    pm, pm_lat = plain_form(C, gate_fn, [(i,) for i in mux_idx], args.plain_streams)
+    achieved = FLOP_PER_BOOTSTRAP_KB_LVL2 * nb_mix / (br_mix_ms / 1e3) / 1e12
+    res = base_result(args, mix["gates_per_s"], mix["ms_per_step"], "f64", {
+        "workload": "batched lvl02 gates (lvl0 in/out, BR at N=2048 Torus64, KB=2, FFT): "
+                    "1536 binary (uniform over NAND/AND/OR/XOR/XNOR/NOR) + 512 MUX, interleaved",
+        "global_batch": n, "n": 636, "N": 2048, "l": 4, "Bgbit": 10,
+        "parallelism": "single GPU",
+    })
+    res["metric"] = "lvl02 gates/s, 1536 binary + 512 MUX, batched vs plain per-gate form"
+    res.update({
+        "decrypt_failures": failures,
+        "per_type": {"mix": mix, "binary": binr, "mux": muxr},
+        "plain_form": {"binary_gates_per_s": round(pb, 2), "mux_gates_per_s": round(pm, 2),
+                       "binary_latency_p50_ms": round(pb_lat, 3),
+                       "mux_latency_p50_ms": round(pm_lat, 3), "streams": args.plain_streams},
+        "batched_over_plain": {"binary": round(binr["gates_per_s"] / pb, 2),
+                               "mux": round(muxr["gates_per_s"] / pm, 2)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                     "traffic": None},
+        "cpu_baseline": None,
+    })
+    return res
+
+
+# ------------------------------------------------------------------ NTT block-binary
+def workload_nttbb(args):
+    import numpy as np
+
+    import cufhe_amd as C
+
+    B = args.batch
+    log("keys (seed 0, block-binary lvl0 key, subset KSK)")
+    k0, k1, _ = C.secret_key_gen(0, block_binary=True)
+    bk = C.bk_gen(1, False, 0, k0, k1)
+    ksk = C.ksk_gen(1, 0, k1, k0, subset=True)
+    C.check(C.lib().cufhe_initialize(), "initialize")
+    C.initialize_lvl01(bk, ksk, backend=C.BACKEND_NTT, variant=C.BR_BLOCK_BINARY, subset=True)
+    rng = np.random.default_rng(args.seed)
+    ops = ["NAND" if v == 0 else "XOR" for v in rng.integers(0, 2, B)]
+    bits = rng.integers(0, 2, 2 * B).astype(np.uint8)
+    pool = np.zeros((3 * B, C.TLWE1_WORDS), np.uint32)
+    pool[: 2 * B] = C.encrypt_bits(1, 3000 + args.seed, k1, bits)
+    dpool = C.DeviceBuffer.from_numpy(pool)
+    gates = [(ops[i], 2 * B + i, i, B + i, 0) for i in range(B)]
+    want = np.array([C.TRUTH[o](int(bits[i]), int(bits[B + i])) for i, o in enumerate(ops)],
+                    np.uint8)
+    cands = [args.jobs_per_group] if args.jobs_per_group else [1, 2, 4]
+    best_c, tune = autotune(C, gates, dpool, cands)
+    plan = C.Plan(gates, dpool, jobs_per_group=best_c)
+    nbr = plan.bootstraps
+    for _ in range(args.warmup):
+        plan.run_timed()
+    out = dpool.download(np.uint32, (3 * B, C.TLWE1_WORDS))[2 * B:]
+    failures = int(np.sum(C.decrypt_bits(1, k1, out) != want))
+    bit_exact = None
+    if not args.no_cpu:
+        import oracle as O
+        sample = rng.choice(B, 32, replace=False)
+        codes = np.array([C.GATES[ops[i]] for i in sample], np.int32)
+        ref = O.gate_batch_lvl1(O.NTT, O.BR_BB | O.SUBSET, codes, pool[sample], pool[B + sample],
+                                bk, ksk, min(16, os.cpu_count() or 1))
+        bit_exact = int(sum(np.array_equal(out[i], ref[j]) for j, i in enumerate(sample)))
+        log("bit-exact vs oracle: %d / %d" % (bit_exact, len(sample)))
+    total, step_ms, ks_ms, br_ms, _ = timed_steps(C, plan, args.steps)
+    value = B * args.steps / total
+    gms = graph_ms(C, gates, dpool, best_c)
+    W = C.TLWE1_WORDS * 4
+
+    def gate_fn(i, st):
+        C.check(C.lib().cufhe_gate_lvl1(C.GATES[ops[i]], dpool.at((2 * B + i) * W),
+                                        dpool.at(i * W), dpool.at((B + i) * W), None, st), "gate")
+    plain_tput, plain_lat = plain_form(C, gate_fn, [(i,) for i in range(min(args.plain_sample, B))],
+                                       args.plain_streams)
+    br_avg_s = statistics.mean(br_ms) / 1e3
+    groups = -(-nbr // best_c)
+    key_gbps = BK_BYTES_NTT_BB * groups / br_avg_s / 1e9
+    res = base_result(args, value, total * 1e3 / args.steps, "u32 mod P", {
+        "workload": "batched lvl1 NAND/XOR gates, NTT backend (P=2147473409), block-binary "
+                    "lvl0 key (blocks of 4), subset key switching, 4096 gates",
+        "global_batch": B, "n": 636, "N": 1024, "l": 2, "Bgbit": 8, "ell": 4,
+        "jobs_per_group": best_c, "autotune_br_ms": {str(k): round(v, 3) for k, v in tune.items()},
+        "parallelism": "single GPU",
+    })
+    res["metric"] = "lvl1 NAND/XOR bootstraps/s @batch 4096, NTT block-binary"
+    res.update({
+        "p50_layer_ms": round(pct(step_ms, 50), 3), "p99_layer_ms": round(pct(step_ms, 99), 3),
+        "graph_ms_per_step": round(gms, 3),
+        "kernel_ms": {"key_switch": round(statistics.mean(ks_ms), 3),
+                      "blind_rotate": round(statistics.mean(br_ms), 3)},
+        "plain_form": {"gates_per_s": round(plain_tput, 2), "streams": args.plain_streams,
+                       "latency_p50_ms": round(plain_lat, 3)},
+        "batched_over_plain": round(value / plain_tput, 2),
+        "decrypt_failures": failures,
+        "bit_exact_sample": bit_exact,
+        "roofline": {"bound": "hbm", "achieved": round(key_gbps, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(key_gbps / HBM_PEAK_GBPS, 4),
+                     "traffic": measured_traffic("nttbb")},
+        "cpu_baseline": None,
+    })
+    return res
+
+
+WORKLOADS = {"kb4096": workload_kb4096, "lvl2mix": workload_lvl2mix, "nttbb": workload_nttbb}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="kb4096")
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--jobs-per-group", type=int, default=0,
+                    help="bootstraps per workgroup; 0 = autotune over {1, 2, 4, 8}")
+    ap.add_argument("--plain-sample", type=int, default=256)
+    ap.add_argument("--plain-streams", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="after the workload, copy this many MiB device-to-device (a kernel of "
+                         "known traffic for scaling HBM counters); not part of any figure")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: the gate batch is one GPU's share; "
+              "multi-GPU sharding (independent gates, replicated keys) is not part of this "
+              "build. Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    res = WORKLOADS[args.workload](args)
+    if args.calibrate_copy_mib > 0:
+        import cufhe_amd as C
+        n = args.calibrate_copy_mib << 20
+        a, b = C.DeviceBuffer(n), C.DeviceBuffer(n)
+        C.check(C.lib().cufhe_memset_d(a.ptr, 1, n, None), "memset")
+        C.check(C.lib().cufhe_synchronize(), "synchronize")
+        C.check(C.lib().cufhe_memcpy_d2d(b.ptr, a.ptr, n, None), "copy")
+        C.check(C.lib().cufhe_synchronize(), "synchronize")
+        log("calibration copy: %d bytes device-to-device" % n)
     print(json.dumps(res), flush=True)
 
 
