@@ -411,7 +411,89 @@ def workload_nttbb(args):
     return res
 
 
-WORKLOADS = {"kb4096": workload_kb4096, "lvl2mix": workload_lvl2mix, "nttbb": workload_nttbb}
+# ------------------------------------------------------------------ serving
+def workload_serving(args):
+    import threading
+
+    import numpy as np
+
+    import cufhe_amd as C
+    from cufhe_amd.circuits import random_circuit
+
+    log("keys (seed 0, key bundle)")
+    k0, k1, _ = C.secret_key_gen(0)
+    bk = C.bk_gen(1, True, 0, k0, k1)
+    ksk = C.ksk_gen(1, 0, k1, k0)
+    C.check(C.lib().cufhe_initialize(), "initialize")
+    C.initialize_lvl01(bk, ksk, backend=C.BACKEND_FFT, variant=C.BR_KEY_BUNDLE)
+    rng = np.random.default_rng(args.seed)
+    rates = [float(r) for r in args.serve_rates.split(",")]
+    levels = []
+    for li, rate in enumerate(rates):
+        circs, ins, bits = [], [], []
+        for i in range(args.serve_circuits):
+            c = random_circuit(rng, int(rng.integers(64, 1025)), int(rng.integers(4, 33)))
+            b = rng.integers(0, 2, c.n_inputs).astype(np.uint8)
+            circs.append(c)
+            bits.append(b)
+            ins.append(C.encrypt_bits(1, 10000 * (li + 1) + i, k1, b))
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, len(circs)))
+        levels.append((rate, circs, ins, bits, arrivals))
+    log("%d circuits per level, rates %s circuits/s" % (args.serve_circuits, rates))
+    results = []
+    for rate, circs, ins, bits, arrivals in levels:
+        srv = C.Server(pool_bytes=int(args.serve_pool_gb * (1 << 30)),
+                       jobs_per_group=args.jobs_per_group)
+        ids = [None] * len(circs)
+        t_start = time.perf_counter()
+
+        def client():
+            for i, c in enumerate(circs):
+                dt = t_start + arrivals[i] - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                ids[i] = srv.submit(c, ins[i])
+
+        th = threading.Thread(target=client)
+        th.start()
+        th.join()
+        outs = [srv.result(r) for r in ids]
+        infos = [srv.info(r) for r in ids]
+        st = srv.stats()
+        srv.destroy()
+        lat = [1e3 * (f["done"] - f["submit"]) for f in infos]
+        span = max(f["done"] for f in infos) - min(f["submit"] for f in infos)
+        gates = sum(c.n_gates for c in circs)
+        fails = sum(int(np.sum(C.decrypt_bits(1, k1, o) != c.evaluate(b)))
+                    for o, c, b in zip(outs, circs, bits))
+        r = {"rate_circuits_per_s": rate, "circuits": len(circs), "gates": gates,
+             "gates_per_s": round(gates / span, 1),
+             "p50_latency_ms": round(pct(lat, 50), 1), "p99_latency_ms": round(pct(lat, 99), 1),
+             "rounds": st["rounds"], "mean_gates_per_round": round(st["gates"] / st["rounds"], 1),
+             "gpu_busy_frac": round(st["busy_s"] / span, 3),
+             "peak_pool_MiB": round(st["peak_pool_bytes"] / 2**20, 1),
+             "decrypt_failures": fails}
+        log("serving %s" % r)
+        results.append(r)
+    top = results[-1]
+    res = base_result(args, top["gates_per_s"], 1e3 / max(top["gates_per_s"], 1e-9), "f64", {
+        "workload": "serving: Poisson arrivals of random layered lvl1 circuits (width U[64,1024], "
+                    "depth U[4,32], 10%% MUX, binary gates uniform), KB=2 FFT, device-memory "
+                    "admission (%g GiB pool), layers of all admitted circuits co-batched per round"
+                    % args.serve_pool_gb,
+        "rates_circuits_per_s": rates, "circuits_per_rate": args.serve_circuits,
+        "parallelism": "single GPU",
+    })
+    res["metric"] = "serving gates/s and p50/p99 per-circuit latency (lvl1 circuits, KB=2)"
+    res.update({"levels": results,
+                "p50_latency_ms": top["p50_latency_ms"], "p99_latency_ms": top["p99_latency_ms"],
+                "decrypt_failures": sum(r["decrypt_failures"] for r in results),
+                "roofline": None, "cpu_baseline": None})
+    return res
+
+
+WORKLOADS = {"kb4096": workload_kb4096, "lvl2mix": workload_lvl2mix, "nttbb": workload_nttbb,
+             "serving": workload_serving}
 
 
 def main():
@@ -429,6 +511,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--serve-rates", default="2,5,10", help="circuits/s, one level each")
+    ap.add_argument("--serve-circuits", type=int, default=24, help="circuits per rate level")
+    ap.add_argument("--serve-pool-gb", type=float, default=4.0)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the workload, copy this many MiB device-to-device (a kernel of "
                          "known traffic for scaling HBM counters); not part of any figure")

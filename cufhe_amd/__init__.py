@@ -54,6 +54,13 @@ class GateDesc(ctypes.Structure):
                 ("in1", ctypes.c_uint32), ("inc", ctypes.c_uint32)]
 
 
+class CircuitDesc(ctypes.Structure):
+    _fields_ = [("n_inputs", ctypes.c_int), ("n_gates", ctypes.c_int),
+                ("gates", ctypes.c_void_p), ("n_layers", ctypes.c_int),
+                ("layer_end", ctypes.c_void_p), ("n_outputs", ctypes.c_int),
+                ("outputs", ctypes.c_void_p)]
+
+
 _lib = None
 
 
@@ -109,6 +116,13 @@ def lib():
         "cufhe_ffnt_inverse": ([i32, vp, vp, i32, vp], i32),
         "cufhe_trgsw_to_fourier": ([i32, vp, vp, i32, vp], i32),
         "cufhe_cmux_accumulate": ([i32, vp, vp, u32, vp], i32),
+        "cufhe_server_create": ([sz, i32, i32, ctypes.POINTER(vp)], i32),
+        "cufhe_server_submit": ([vp, vp, vp, vp, ctypes.POINTER(u64)], i32),
+        "cufhe_server_wait": ([vp, u64, i32], i32),
+        "cufhe_server_request_info": ([vp, u64, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(i32)], i32),
+        "cufhe_server_stats": ([vp, ctypes.POINTER(ctypes.c_double)], i32),
+        "cufhe_server_destroy": ([vp], i32),
         "tfhe_secret_key_gen": ([u64, i32, vp, vp, vp], i32),
         "tfhe_bk_gen": ([i32, i32, u64, vp, vp, vp, sz, ctypes.POINTER(sz)], i32),
         "tfhe_ksk_gen": ([i32, u64, vp, vp, i32, vp, sz], i32),
@@ -320,6 +334,74 @@ class Plan:
         if self.handle is not None:
             lib().cufhe_plan_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching server of layered lvl1 circuits (cufhe_server_*)."""
+
+    def __init__(self, pool_bytes, max_batch_bootstraps=8192, jobs_per_group=0):
+        h = ctypes.c_void_p()
+        check(lib().cufhe_server_create(int(pool_bytes), int(max_batch_bootstraps),
+                                        int(jobs_per_group), ctypes.byref(h)),
+              "cufhe_server_create")
+        self.handle = h
+        self._outputs = {}
+
+    def submit(self, circuit, inputs):
+        """circuit: circuits.Circuit; inputs: (n_inputs, 1025) uint32 TLWEs.
+        Returns the request id; outputs via result(id) once done."""
+        g = np.zeros((circuit.n_gates, 5), np.uint32)
+        g[:, 0] = circuit.ops
+        g[:, 2] = circuit.in0
+        g[:, 3] = circuit.in1
+        g[:, 4] = circuit.inc
+        layer_end = np.ascontiguousarray(circuit.layer_end, np.int32)
+        outs = np.ascontiguousarray(circuit.outputs, np.uint32)
+        inputs = np.ascontiguousarray(inputs, np.uint32)
+        assert inputs.shape == (circuit.n_inputs, TLWE1_WORDS)
+        out = np.zeros((len(outs), TLWE1_WORDS), np.uint32)
+        desc = CircuitDesc(circuit.n_inputs, circuit.n_gates, g.ctypes.data, len(layer_end),
+                           layer_end.ctypes.data, len(outs), outs.ctypes.data)
+        rid = ctypes.c_uint64()
+        check(lib().cufhe_server_submit(self.handle, ctypes.byref(desc), inputs.ctypes.data,
+                                        out.ctypes.data, ctypes.byref(rid)), "cufhe_server_submit")
+        self._outputs[rid.value] = out
+        return rid.value
+
+    def wait(self, rid, timeout_ms=-1):
+        rc = lib().cufhe_server_wait(self.handle, rid, int(timeout_ms))
+        if rc < 0:
+            check(rc, "cufhe_server_wait")
+        return rc == 1
+
+    def result(self, rid):
+        self.wait(rid)
+        return self._outputs.pop(rid)
+
+    def info(self, rid):
+        t = (ctypes.c_double * 3)()
+        rounds = ctypes.c_int()
+        check(lib().cufhe_server_request_info(self.handle, rid, t, ctypes.byref(rounds)),
+              "cufhe_server_request_info")
+        return {"submit": t[0], "admit": t[1], "done": t[2], "rounds": rounds.value}
+
+    def stats(self):
+        st = (ctypes.c_double * 5)()
+        check(lib().cufhe_server_stats(self.handle, st), "cufhe_server_stats")
+        return {"rounds": int(st[0]), "bootstraps": int(st[1]), "gates": int(st[2]),
+                "busy_s": st[3], "peak_pool_bytes": int(st[4])}
+
+    def destroy(self):
+        if self.handle is not None:
+            rc = lib().cufhe_server_destroy(self.handle)
+            self.handle = None
+            check(rc, "cufhe_server_destroy")
 
     def __del__(self):
         try:

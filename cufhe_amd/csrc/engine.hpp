@@ -1,0 +1,48 @@
+// Internal engine interface shared by the C ABI's plans (runtime.hip) and the
+// serving loop (server.hip): turning a lvl1 gate list into key-switch,
+// blind-rotation and combine jobs over a device wire pool, and launching
+// them on the loaded lvl1 keys.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/cufhe_amd.h"
+#include "device_common.hpp"
+
+namespace cufhe_amd {
+
+// Records msg as the calling thread's last error and returns code.
+int set_error(int code, const std::string& msg);
+
+constexpr uint32_t kWireWords1 = lvl1param::n + 1;  // lvl1 TLWE words per wire
+constexpr uint32_t kLvl0Stride = 640;                // lvl0 TLWE stride in workspaces
+
+struct Lvl1Jobs {
+    std::vector<KsJob> ks;
+    std::vector<BrJob> br;
+    std::vector<LinJob> lin;
+};
+
+// Key switches (= bootstraps) and MUX/NMUX gates of a gate list; fails on an
+// unknown op.
+int lvl1_count(const cufhe_gate_desc* gates, int n, int* nks, int* nmux);
+// Workspace words: one lvl0 TLWE per bootstrap plus two lvl1 TLWEs per MUX.
+inline size_t lvl1_workspace_words(int nks, int nmux)
+{
+    return static_cast<size_t>(nks) * kLvl0Stride + static_cast<size_t>(nmux) * 2 * kWireWords1;
+}
+// Appends the jobs of gates[0, n) (wire w at d_pool + w * kWireWords1) to
+// jobs, using ws (lvl1_workspace_words of this list) for intermediates.
+void lvl1_build(const cufhe_gate_desc* gates, int n, uint32_t* d_pool, uint32_t* ws, Lvl1Jobs& jobs);
+bool lvl1_loaded();
+// Validated jobs-per-workgroup (0 = default), or -1.
+int lvl1_pick_c(int c);
+// Key switch -> blind rotation + SampleExtract -> combine, on stream st.
+void lvl1_launch(const KsJob* ks, int nks, const BrJob* br, int nbr, const LinJob* lin, int nlin,
+                 int c, hipStream_t st);
+
+}  // namespace cufhe_amd

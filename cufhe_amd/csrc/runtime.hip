@@ -14,18 +14,22 @@
 #include "../../include/tfhe_params.hpp"
 #include "br_fft.hpp"
 #include "br_ntt.hpp"
+#include "engine.hpp"
 #include "launch.hpp"
 
 namespace cufhe_amd {
-namespace {
 
 thread_local std::string g_last_error;
 
-int fail(int code, const std::string& msg)
+int set_error(int code, const std::string& msg)
 {
     g_last_error = msg;
     return code;
 }
+
+namespace {
+
+int fail(int code, const std::string& msg) { return set_error(code, msg); }
 
 #define CUFHE_HIP_TRY(expr)                                                              \
     do {                                                                                 \
@@ -391,6 +395,92 @@ void gate_lvl1_one(bool mux, uint32_t* out, const uint32_t* in0, const uint32_t*
 }
 
 }  // namespace
+
+int lvl1_count(const cufhe_gate_desc* gates, int n, int* nks, int* nmux)
+{
+    int k = 0, m = 0;
+    for (int i = 0; i < n; i++) {
+        const int op = gates[i].op;
+        if (op < 0 || op >= CUFHE_GATE_COUNT)
+            return set_error(CUFHE_ERR_ARG, "unknown gate op " + std::to_string(op));
+        if (op == CUFHE_MUX || op == CUFHE_NMUX) {
+            k += 2;
+            m++;
+        }
+        else if (op != CUFHE_NOT && op != CUFHE_COPY) {
+            k += 1;
+        }
+    }
+    *nks = k;
+    *nmux = m;
+    return CUFHE_OK;
+}
+
+// Gate list -> jobs (reference src/bootstrap_gpu.cu:1223-1246 IKS-BR gates,
+// :2069-2197 MUX, :2200-2222 NOT/COPY): a binary gate is one key switch of
+// ca*in0 + cb*in1 + offset followed by one rotation; MUX/NMUX are two
+// (c + t - mu, -c + f - mu) combined as +-(r1 + r0 + mu); NOT/COPY are
+// coefficient-wise.
+void lvl1_build(const cufhe_gate_desc* gates, int n, uint32_t* d_pool, uint32_t* ws, Lvl1Jobs& jobs)
+{
+    constexpr uint32_t W1 = kWireWords1, W0 = kLvl0Stride;
+    constexpr uint32_t mu = static_cast<uint32_t>(lvl1param::μ);
+    int nks = 0, nmux = 0;
+    (void)lvl1_count(gates, n, &nks, &nmux);
+    uint32_t* lvl0 = ws;
+    uint32_t* res = ws + static_cast<size_t>(nks) * W0;
+    int ks_i = 0, mux_i = 0;
+    auto wire = [&](uint32_t w) { return d_pool + static_cast<size_t>(w) * W1; };
+    for (int i = 0; i < n; i++) {
+        const cufhe_gate_desc& gd = gates[i];
+        if (gd.op == CUFHE_NOT || gd.op == CUFHE_COPY) {
+            jobs.lin.push_back(LinJob{wire(gd.in0), nullptr, wire(gd.out),
+                                      gd.op == CUFHE_NOT ? -1 : 1, 0, 0u, static_cast<int32_t>(W1)});
+            continue;
+        }
+        if (gd.op == CUFHE_MUX || gd.op == CUFHE_NMUX) {
+            uint32_t* r1 = res + static_cast<size_t>(2 * mux_i) * W1;
+            uint32_t* r0 = r1 + W1;
+            mux_i++;
+            const int sgn = gd.op == CUFHE_MUX ? 1 : -1;
+            for (int half = 0; half < 2; half++) {
+                uint32_t* l0 = lvl0 + static_cast<size_t>(ks_i++) * W0;
+                jobs.ks.push_back(KsJob{wire(gd.inc), wire(half == 0 ? gd.in1 : gd.in0), l0,
+                                        half == 0 ? 1 : -1, 1, static_cast<uint64_t>(0u - mu)});
+                jobs.br.push_back(BrJob{l0, nullptr, half == 0 ? r1 : r0, 1, 0, 0u, 0u});
+            }
+            jobs.lin.push_back(LinJob{r1, r0, wire(gd.out), sgn, sgn,
+                                      static_cast<uint32_t>(sgn) * mu, static_cast<int32_t>(W1)});
+            continue;
+        }
+        GateLin gl;
+        gate_lin(gd.op, gl);
+        uint32_t* l0 = lvl0 + static_cast<size_t>(ks_i++) * W0;
+        jobs.ks.push_back(KsJob{wire(gd.in0), wire(gd.in1), l0, gl.ca, gl.cb,
+                                static_cast<uint64_t>(static_cast<uint32_t>(gl.off_mu) * mu)});
+        jobs.br.push_back(BrJob{l0, nullptr, wire(gd.out), 1, 0, 0u, 0u});
+    }
+}
+
+bool lvl1_loaded() { return state().k1.loaded; }
+
+int lvl1_pick_c(int c)
+{
+    // default from the MI355X sweep of {1, 2, 4, 8} (lvl1, 4096 gates):
+    // one job per workgroup wins once the key stream is L2-resident
+    if (c == 0) c = 1;
+    if (c != 1 && c != 2 && c != 4 && c != 8) return -1;
+    return c;
+}
+
+void lvl1_launch(const KsJob* ks, int nks, const BrJob* br, int nbr, const LinJob* lin, int nlin,
+                 int c, hipStream_t st)
+{
+    ks_lvl1(ks, nks, st);
+    br_lvl1(br, nbr, c, static_cast<uint32_t>(lvl1param::μ), st);
+    if (nlin) launch_linear(lin, nlin, st);
+}
+
 }  // namespace cufhe_amd
 
 using namespace cufhe_amd;
@@ -429,23 +519,15 @@ int plan_launch(cufhe_plan* p, hipStream_t st)
         if (p->njobs_lin) launch_linear(p->lin, p->njobs_lin, st);
         return check_launch("batched lvl02 gate launch");
     }
-    ks_lvl1(p->ks, p->njobs_ks, st);
-    br_lvl1(p->br, p->njobs_br, p->c, static_cast<uint32_t>(lvl1param::μ), st);
-    if (p->njobs_lin) launch_linear(p->lin, p->njobs_lin, st);
+    lvl1_launch(p->ks, p->njobs_ks, p->br, p->njobs_br, p->lin, p->njobs_lin, p->c, st);
     return check_launch("batched gate launch");
 }
 
 int pick_c(int c, bool kb, int nbr)
 {
-    if (c == 0) {
-        // default from the MI355X sweep of {1, 2, 4, 8} (lvl1, 4096 gates):
-        // one job per workgroup wins once the key stream is L2-resident
-        c = 1;
-        (void)nbr;
-    }
-    if (c != 1 && c != 2 && c != 4 && c != 8) return -1;
     (void)kb;
-    return c;
+    (void)nbr;
+    return lvl1_pick_c(c);
 }
 
 }  // namespace
@@ -706,81 +788,35 @@ int cufhe_plan_create_lvl1(const cufhe_gate_desc* gates, int n, uint32_t* d_pool
     *out = nullptr;
     if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded (cufhe_initialize_lvl01)");
     if (n < 0 || (n > 0 && (!gates || !d_pool))) return fail(CUFHE_ERR_ARG, "bad gate list");
-    constexpr uint32_t W1 = lvl1param::n + 1;
-    constexpr uint32_t W0 = 640;  // lvl0 TLWE stride in the workspace
-    constexpr uint32_t mu = static_cast<uint32_t>(lvl1param::μ);
     int nks = 0, nmux = 0;
-    for (int i = 0; i < n; i++) {
-        const int op = gates[i].op;
-        if (op < 0 || op >= CUFHE_GATE_COUNT) return fail(CUFHE_ERR_ARG, "unknown gate op");
-        if (op == CUFHE_MUX || op == CUFHE_NMUX) {
-            nks += 2;
-            nmux++;
-        }
-        else if (op != CUFHE_NOT && op != CUFHE_COPY) {
-            nks += 1;
-        }
-    }
+    int rc = lvl1_count(gates, n, &nks, &nmux);
+    if (rc != CUFHE_OK) return rc;
     const int c = pick_c(jobs_per_group, s.k1.kb, nks);
     if (c < 0) return fail(CUFHE_ERR_UNSUPPORTED, "jobs_per_group must be 0, 1, 2, 4 or 8");
     cufhe_plan* p = new cufhe_plan();
     p->c = c;
     p->use_graph = use_graph != 0;
-    const size_t ws_words = static_cast<size_t>(nks) * W0 + static_cast<size_t>(nmux) * 2 * W1;
+    const size_t ws_words = lvl1_workspace_words(nks, nmux);
     if (ws_words) {
         if (hipMalloc(reinterpret_cast<void**>(&p->ws), ws_words * sizeof(uint32_t)) != hipSuccess) {
             delete p;
             return fail(CUFHE_ERR_NOMEM, "plan workspace allocation failed");
         }
     }
-    std::vector<KsJob> ks;
-    std::vector<BrJob> br;
-    std::vector<LinJob> lin;
-    ks.reserve(nks);
-    br.reserve(nks);
-    uint32_t* lvl0 = p->ws;
-    uint32_t* res = p->ws + static_cast<size_t>(nks) * W0;
-    int mux_i = 0;
-    auto wire = [&](uint32_t w) { return d_pool + static_cast<size_t>(w) * W1; };
-    for (int i = 0; i < n; i++) {
-        const cufhe_gate_desc& gd = gates[i];
-        if (gd.op == CUFHE_NOT || gd.op == CUFHE_COPY) {
-            lin.push_back(LinJob{wire(gd.in0), nullptr, wire(gd.out), gd.op == CUFHE_NOT ? -1 : 1, 0,
-                                 0u, static_cast<int32_t>(W1)});
-            continue;
-        }
-        if (gd.op == CUFHE_MUX || gd.op == CUFHE_NMUX) {
-            uint32_t* r1 = res + static_cast<size_t>(2 * mux_i) * W1;
-            uint32_t* r0 = r1 + W1;
-            mux_i++;
-            const int sgn = gd.op == CUFHE_MUX ? 1 : -1;
-            for (int half = 0; half < 2; half++) {
-                uint32_t* l0 = lvl0 + static_cast<size_t>(ks.size()) * W0;
-                ks.push_back(KsJob{wire(gd.inc), wire(half == 0 ? gd.in1 : gd.in0), l0,
-                                   half == 0 ? 1 : -1, 1, static_cast<uint64_t>(0u - mu)});
-                br.push_back(BrJob{l0, nullptr, half == 0 ? r1 : r0, 1, 0, 0u, 0u});
-            }
-            lin.push_back(LinJob{r1, r0, wire(gd.out), sgn, sgn, static_cast<uint32_t>(sgn) * mu,
-                                 static_cast<int32_t>(W1)});
-            continue;
-        }
-        GateLin gl;
-        gate_lin(gd.op, gl);
-        uint32_t* l0 = lvl0 + static_cast<size_t>(ks.size()) * W0;
-        ks.push_back(KsJob{wire(gd.in0), wire(gd.in1), l0, gl.ca, gl.cb,
-                           static_cast<uint64_t>(static_cast<uint32_t>(gl.off_mu) * mu)});
-        br.push_back(BrJob{l0, nullptr, wire(gd.out), 1, 0, 0u, 0u});
-    }
-    int rc = alloc_upload(&p->ks, ks);
-    if (rc == CUFHE_OK) rc = alloc_upload(&p->br, br);
-    if (rc == CUFHE_OK) rc = alloc_upload(&p->lin, lin);
+    Lvl1Jobs jobs;
+    jobs.ks.reserve(nks);
+    jobs.br.reserve(nks);
+    lvl1_build(gates, n, d_pool, p->ws, jobs);
+    rc = alloc_upload(&p->ks, jobs.ks);
+    if (rc == CUFHE_OK) rc = alloc_upload(&p->br, jobs.br);
+    if (rc == CUFHE_OK) rc = alloc_upload(&p->lin, jobs.lin);
     if (rc != CUFHE_OK) {
         cufhe_plan_destroy(p);
         return rc;
     }
-    p->njobs_ks = static_cast<int>(ks.size());
-    p->njobs_br = static_cast<int>(br.size());
-    p->njobs_lin = static_cast<int>(lin.size());
+    p->njobs_ks = static_cast<int>(jobs.ks.size());
+    p->njobs_br = static_cast<int>(jobs.br.size());
+    p->njobs_lin = static_cast<int>(jobs.lin.size());
     *out = p;
     return CUFHE_OK;
 }

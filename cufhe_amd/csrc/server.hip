@@ -1,0 +1,416 @@
+// Serving loop behind cufhe_server_* (include/cufhe_amd.h): continuous
+// batching of layered lvl1 Boolean circuits on one GPU.
+//
+// Data flow. The server owns one device wire pool (lvl1 TLWEs of 1025 words)
+// carved by a first-fit range allocator. A submitted circuit waits in a FIFO
+// until its wires fit; at admission its inputs are copied into its range and
+// every later layer's outputs are written next to them, so ciphertexts stay
+// in HBM for the circuit's whole life. Each round the worker thread takes
+// the next layer of every admitted circuit (in admission order, whole layers,
+// up to the round's bootstrap and gate budget), rewrites their local wire ids
+// to pool slots, and runs them as one batch: key switch -> blind rotation ->
+// combine (engine.hpp). Circuits whose last layer ran get their outputs
+// copied back and their range freed, which admits waiting circuits in the
+// next round. Gates never depend on their batch-mates, so every result is
+// bit-identical to running the circuit alone.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "engine.hpp"
+
+using namespace cufhe_amd;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// First-fit allocator of contiguous wire ranges with coalescing.
+class WireArena {
+   public:
+    void reset(size_t wires)
+    {
+        free_.clear();
+        if (wires) free_[0] = wires;
+        used_ = 0;
+    }
+    bool alloc(size_t n, size_t* base)
+    {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            *base = it->first;
+            const size_t rest = it->second - n, start = it->first + n;
+            free_.erase(it);
+            if (rest) free_[start] = rest;
+            used_ += n;
+            return true;
+        }
+        return false;
+    }
+    void release(size_t base, size_t n)
+    {
+        auto it = free_.emplace(base, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+        used_ -= n;
+    }
+    size_t used() const { return used_; }
+
+   private:
+    std::map<size_t, size_t> free_;
+    size_t used_ = 0;
+};
+
+struct Request {
+    uint64_t id = 0;
+    int n_inputs = 0, n_gates = 0, n_layers = 0;
+    std::vector<cufhe_gate_desc> gates;
+    std::vector<int> layer_end;
+    std::vector<int> layer_boot;  // bootstraps per layer
+    std::vector<uint32_t> outputs;
+    std::vector<uint32_t> inputs;  // n_inputs x 1025 words
+    uint32_t* host_out = nullptr;
+    size_t wires = 0, base = 0;
+    int next_layer = 0, rounds = 0;
+    int status = 0;  // 0 running, 1 done, < 0 error code
+    double t_submit = 0, t_admit = -1, t_done = -1;
+};
+using ReqPtr = std::shared_ptr<Request>;
+
+}  // namespace
+
+struct cufhe_server {
+    uint32_t* pool = nullptr;
+    size_t pool_wires = 0;
+    WireArena arena;
+    int max_boot = 0, c = 1;
+    hipStream_t st = nullptr;
+    KsJob* d_ks = nullptr;
+    BrJob* d_br = nullptr;
+    LinJob* d_lin = nullptr;
+    uint32_t* d_ws = nullptr;
+    KsJob* h_ks = nullptr;
+    BrJob* h_br = nullptr;
+    LinJob* h_lin = nullptr;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<ReqPtr> waiting;
+    std::vector<ReqPtr> active;
+    std::unordered_map<uint64_t, ReqPtr> all;
+    uint64_t next_id = 1;
+    bool stop = false;
+    int error = 0;
+    std::string error_msg;
+    Clock::time_point t0;
+    double stats[5] = {0, 0, 0, 0, 0};
+    std::thread worker;
+
+    double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+};
+
+namespace {
+
+void fail_all_locked(cufhe_server* S, int code, const std::string& msg)
+{
+    S->error = code;
+    S->error_msg = msg;
+    for (auto& r : S->active) r->status = code;
+    for (auto& r : S->waiting) r->status = code;
+    S->active.clear();
+    S->waiting.clear();
+    S->cv_done.notify_all();
+}
+
+// One round: admit, collect the next layer of every admitted circuit,
+// launch them as one batch, retire finished circuits.
+void worker_loop(cufhe_server* S)
+{
+    constexpr uint32_t W1 = kWireWords1;
+    for (;;) {
+        std::vector<ReqPtr> admitted_now, finishing;
+        std::vector<cufhe_gate_desc> gates;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_work.wait(lk, [&] { return S->stop || !S->waiting.empty() || !S->active.empty(); });
+            if (S->waiting.empty() && S->active.empty()) {
+                if (S->stop) return;
+                continue;
+            }
+            while (!S->waiting.empty()) {
+                ReqPtr r = S->waiting.front();
+                size_t base = 0;
+                if (!S->arena.alloc(r->wires, &base)) break;
+                r->base = base;
+                r->t_admit = S->now();
+                S->active.push_back(r);
+                admitted_now.push_back(r);
+                S->waiting.pop_front();
+            }
+            S->stats[4] = std::max(S->stats[4], static_cast<double>(S->arena.used()) * W1 * 4);
+            int boots = 0, ng = 0;
+            for (auto& r : S->active) {
+                const int l = r->next_layer;
+                const int g0 = l == 0 ? 0 : r->layer_end[l - 1], g1 = r->layer_end[l];
+                const int lb = r->layer_boot[l], lg = g1 - g0;
+                if (!gates.empty() && (boots + lb > S->max_boot || ng + lg > S->max_boot)) continue;
+                const uint32_t b = static_cast<uint32_t>(r->base);
+                for (int g = g0; g < g1; g++) {
+                    cufhe_gate_desc d = r->gates[g];
+                    d.out = b + static_cast<uint32_t>(r->n_inputs + g);
+                    d.in0 += b;
+                    d.in1 += b;
+                    d.inc += b;
+                    gates.push_back(d);
+                }
+                boots += lb;
+                ng += lg;
+                r->next_layer++;
+                r->rounds++;
+                if (r->next_layer == r->n_layers) finishing.push_back(r);
+            }
+            S->stats[0] += 1;
+            S->stats[1] += boots;
+            S->stats[2] += ng;
+        }
+        const double tb = S->now();
+        hipError_t e = hipSuccess;
+        for (auto& r : admitted_now) {
+            if (e != hipSuccess) break;
+            e = hipMemcpyAsync(S->pool + r->base * W1, r->inputs.data(),
+                               r->inputs.size() * sizeof(uint32_t), hipMemcpyHostToDevice, S->st);
+        }
+        Lvl1Jobs jobs;
+        jobs.ks.reserve(S->max_boot);
+        jobs.br.reserve(S->max_boot);
+        lvl1_build(gates.data(), static_cast<int>(gates.size()), S->pool, S->d_ws, jobs);
+        const int nks = static_cast<int>(jobs.ks.size()), nbr = static_cast<int>(jobs.br.size());
+        const int nlin = static_cast<int>(jobs.lin.size());
+        std::memcpy(S->h_ks, jobs.ks.data(), nks * sizeof(KsJob));
+        std::memcpy(S->h_br, jobs.br.data(), nbr * sizeof(BrJob));
+        std::memcpy(S->h_lin, jobs.lin.data(), nlin * sizeof(LinJob));
+        if (e == hipSuccess && nks)
+            e = hipMemcpyAsync(S->d_ks, S->h_ks, nks * sizeof(KsJob), hipMemcpyHostToDevice, S->st);
+        if (e == hipSuccess && nbr)
+            e = hipMemcpyAsync(S->d_br, S->h_br, nbr * sizeof(BrJob), hipMemcpyHostToDevice, S->st);
+        if (e == hipSuccess && nlin)
+            e = hipMemcpyAsync(S->d_lin, S->h_lin, nlin * sizeof(LinJob), hipMemcpyHostToDevice, S->st);
+        if (e == hipSuccess) {
+            lvl1_launch(S->d_ks, nks, S->d_br, nbr, S->d_lin, nlin, S->c, S->st);
+            e = hipGetLastError();
+        }
+        for (auto& r : finishing) {
+            for (size_t o = 0; o < r->outputs.size() && e == hipSuccess; o++)
+                e = hipMemcpyAsync(r->host_out + o * W1, S->pool + (r->base + r->outputs[o]) * W1,
+                                   W1 * sizeof(uint32_t), hipMemcpyDeviceToHost, S->st);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(S->st);
+        const double te = S->now();
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stats[3] += te - tb;
+        if (e != hipSuccess) {
+            fail_all_locked(S, CUFHE_ERR_HIP, std::string("server round: ") + hipGetErrorString(e));
+            continue;
+        }
+        for (auto& r : finishing) {
+            r->status = 1;
+            r->t_done = te;
+            S->arena.release(r->base, r->wires);
+            S->active.erase(std::find(S->active.begin(), S->active.end(), r));
+            // keep the timestamps, drop the circuit
+            std::vector<cufhe_gate_desc>().swap(r->gates);
+            std::vector<uint32_t>().swap(r->inputs);
+        }
+        if (!finishing.empty()) S->cv_done.notify_all();
+    }
+}
+
+void free_server(cufhe_server* S)
+{
+    (void)hipFree(S->pool);
+    (void)hipFree(S->d_ks);
+    (void)hipFree(S->d_br);
+    (void)hipFree(S->d_lin);
+    (void)hipFree(S->d_ws);
+    (void)hipHostFree(S->h_ks);
+    (void)hipHostFree(S->h_br);
+    (void)hipHostFree(S->h_lin);
+    if (S->st) (void)hipStreamDestroy(S->st);
+    delete S;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cufhe_server_create(size_t pool_bytes, int max_batch_bootstraps, int jobs_per_group,
+                        cufhe_server** server)
+{
+    if (!server) return set_error(CUFHE_ERR_ARG, "null server handle");
+    *server = nullptr;
+    if (!lvl1_loaded()) return set_error(CUFHE_ERR_STATE, "lvl1 keys not loaded (cufhe_initialize_lvl01)");
+    const int c = lvl1_pick_c(jobs_per_group);
+    if (c < 0) return set_error(CUFHE_ERR_UNSUPPORTED, "jobs_per_group must be 0, 1, 2, 4 or 8");
+    if (max_batch_bootstraps <= 0) max_batch_bootstraps = 8192;
+    const size_t wire_bytes = static_cast<size_t>(kWireWords1) * sizeof(uint32_t);
+    if (pool_bytes < wire_bytes) return set_error(CUFHE_ERR_ARG, "pool smaller than one ciphertext");
+    auto* S = new cufhe_server();
+    S->pool_wires = pool_bytes / wire_bytes;
+    S->max_boot = max_batch_bootstraps;
+    S->c = c;
+    const size_t mb = static_cast<size_t>(max_batch_bootstraps);
+    hipError_t e = hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&S->pool), S->pool_wires * wire_bytes);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&S->d_ks), mb * sizeof(KsJob));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&S->d_br), mb * sizeof(BrJob));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&S->d_lin), mb * sizeof(LinJob));
+    if (e == hipSuccess)
+        e = hipMalloc(reinterpret_cast<void**>(&S->d_ws),
+                      lvl1_workspace_words(max_batch_bootstraps, max_batch_bootstraps / 2) *
+                          sizeof(uint32_t));
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&S->h_ks), mb * sizeof(KsJob), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&S->h_br), mb * sizeof(BrJob), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&S->h_lin), mb * sizeof(LinJob), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        free_server(S);
+        return set_error(CUFHE_ERR_NOMEM, std::string("server allocation: ") + hipGetErrorString(e));
+    }
+    S->arena.reset(S->pool_wires);
+    S->t0 = Clock::now();
+    S->worker = std::thread(worker_loop, S);
+    *server = S;
+    return CUFHE_OK;
+}
+
+int cufhe_server_submit(cufhe_server* S, const cufhe_circuit* c, const uint32_t* host_inputs,
+                        uint32_t* host_outputs, uint64_t* request_id)
+{
+    if (!S || !c || !request_id) return set_error(CUFHE_ERR_ARG, "null argument");
+    if (c->n_inputs < 0 || c->n_gates <= 0 || c->n_layers <= 0 || c->n_outputs <= 0 || !c->gates ||
+        !c->layer_end || !c->outputs || (c->n_inputs && !host_inputs) || !host_outputs)
+        return set_error(CUFHE_ERR_ARG, "bad circuit");
+    auto r = std::make_shared<Request>();
+    r->n_inputs = c->n_inputs;
+    r->n_gates = c->n_gates;
+    r->n_layers = c->n_layers;
+    r->gates.assign(c->gates, c->gates + c->n_gates);
+    r->layer_end.assign(c->layer_end, c->layer_end + c->n_layers);
+    r->outputs.assign(c->outputs, c->outputs + c->n_outputs);
+    r->wires = static_cast<size_t>(c->n_inputs) + c->n_gates;
+    r->host_out = host_outputs;
+    // validate layers and operands: a gate reads only inputs and earlier layers
+    int prev = 0;
+    for (int l = 0; l < c->n_layers; l++) {
+        const int end = r->layer_end[l];
+        if (end <= prev || end > c->n_gates || (l == c->n_layers - 1 && end != c->n_gates))
+            return set_error(CUFHE_ERR_ARG, "layer_end must increase to n_gates");
+        const uint32_t limit = static_cast<uint32_t>(c->n_inputs + prev);
+        int boots = 0;
+        for (int g = prev; g < end; g++) {
+            cufhe_gate_desc& d = r->gates[g];
+            const int op = d.op;
+            if (op < 0 || op >= CUFHE_GATE_COUNT) return set_error(CUFHE_ERR_ARG, "unknown gate op");
+            const bool mux = op == CUFHE_MUX || op == CUFHE_NMUX;
+            const bool unary = op == CUFHE_NOT || op == CUFHE_COPY;
+            if (!unary && d.in1 >= limit) return set_error(CUFHE_ERR_ARG, "operand from the same or a later layer");
+            if (d.in0 >= limit || (mux && d.inc >= limit))
+                return set_error(CUFHE_ERR_ARG, "operand from the same or a later layer");
+            if (unary) d.in1 = 0;
+            if (!mux) d.inc = 0;
+            boots += mux ? 2 : (unary ? 0 : 1);
+        }
+        if (boots > S->max_boot || end - prev > S->max_boot)
+            return set_error(CUFHE_ERR_SIZE, "a layer exceeds the server's batch budget");
+        r->layer_boot.push_back(boots);
+        prev = end;
+    }
+    for (uint32_t o : r->outputs)
+        if (o >= r->wires) return set_error(CUFHE_ERR_ARG, "output wire out of range");
+    if (r->wires > S->pool_wires) return set_error(CUFHE_ERR_SIZE, "circuit larger than the wire pool");
+    r->inputs.assign(host_inputs, host_inputs + static_cast<size_t>(c->n_inputs) * kWireWords1);
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->error) return set_error(S->error, S->error_msg);
+    if (S->stop) return set_error(CUFHE_ERR_STATE, "server is shutting down");
+    r->id = S->next_id++;
+    r->t_submit = S->now();
+    S->all[r->id] = r;
+    S->waiting.push_back(r);
+    *request_id = r->id;
+    S->cv_work.notify_one();
+    return CUFHE_OK;
+}
+
+int cufhe_server_wait(cufhe_server* S, uint64_t id, int timeout_ms)
+{
+    if (!S) return set_error(CUFHE_ERR_ARG, "null server");
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->all.find(id);
+    if (it == S->all.end()) return set_error(CUFHE_ERR_ARG, "unknown request");
+    ReqPtr r = it->second;
+    auto done = [&] { return r->status != 0; };
+    if (timeout_ms < 0)
+        S->cv_done.wait(lk, done);
+    else if (!S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), done))
+        return 0;
+    if (r->status < 0) return set_error(r->status, S->error_msg);
+    return 1;
+}
+
+int cufhe_server_request_info(cufhe_server* S, uint64_t id, double* times, int* rounds)
+{
+    if (!S) return set_error(CUFHE_ERR_ARG, "null server");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->all.find(id);
+    if (it == S->all.end()) return set_error(CUFHE_ERR_ARG, "unknown request");
+    if (times) {
+        times[0] = it->second->t_submit;
+        times[1] = it->second->t_admit;
+        times[2] = it->second->t_done;
+    }
+    if (rounds) *rounds = it->second->rounds;
+    return CUFHE_OK;
+}
+
+int cufhe_server_stats(cufhe_server* S, double* stats)
+{
+    if (!S || !stats) return set_error(CUFHE_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    for (int i = 0; i < 5; i++) stats[i] = S->stats[i];
+    return CUFHE_OK;
+}
+
+int cufhe_server_destroy(cufhe_server* S)
+{
+    if (!S) return CUFHE_OK;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv_work.notify_all();
+    if (S->worker.joinable()) S->worker.join();
+    const int rc = S->error;
+    free_server(S);
+    return rc;
+}
+
+}  // extern "C"

@@ -171,6 +171,45 @@ int cufhe_plan_run_timed(cufhe_plan* plan, void* stream, float* kernel_ms);
 int cufhe_gate_batch_lvl1(const cufhe_gate_desc* gates, int n, uint32_t* d_pool,
                           int jobs_per_group, void* stream);
 
+/* ---- serving loop (no reference counterpart) -----------------------------
+ * A server owns a device wire pool of pool_bytes and a thread that runs
+ * continuous batching over submitted lvl1 Boolean circuits: a circuit is
+ * admitted (FIFO) once its wires fit in the pool, its inputs are copied in
+ * and stay resident until it completes; every round launches ONE batch made
+ * of the next layer of every admitted circuit (up to max_batch_bootstraps
+ * bootstraps; a layer is never split), and circuits that arrive meanwhile
+ * join the following round. Results are bit-identical to running each
+ * circuit alone (every gate is computed independently of its batch).
+ * Uses the lvl1 keys loaded at submit time; do not reload keys while a
+ * server has work. */
+typedef struct cufhe_server cufhe_server;
+typedef struct {
+    int n_inputs;                  /* wires [0, n_inputs) are the inputs */
+    int n_gates;                   /* gate g writes wire n_inputs + g ('out' ignored) */
+    const cufhe_gate_desc* gates;  /* in topological (layer) order */
+    int n_layers;                  /* gates [layer_end[l-1], layer_end[l]) form layer l; */
+    const int* layer_end;          /*   a gate reads only inputs and earlier layers */
+    int n_outputs;
+    const uint32_t* outputs;       /* output wire ids */
+} cufhe_circuit;
+int cufhe_server_create(size_t pool_bytes, int max_batch_bootstraps, int jobs_per_group,
+                        cufhe_server** server);
+/* Copies the circuit and its n_inputs x 1025-word host inputs; host_outputs
+ * (n_outputs x 1025 words) is written when the request completes and must
+ * stay valid until cufhe_server_wait reports it done. */
+int cufhe_server_submit(cufhe_server* server, const cufhe_circuit* circuit,
+                        const uint32_t* host_inputs, uint32_t* host_outputs, uint64_t* request_id);
+/* 1 = done, 0 = still running after timeout_ms (< 0: wait forever), < 0 error. */
+int cufhe_server_wait(cufhe_server* server, uint64_t request_id, int timeout_ms);
+/* times[3] = submit, admit, done in seconds on the server's clock; rounds =
+ * batches the request took part in. */
+int cufhe_server_request_info(cufhe_server* server, uint64_t request_id, double* times,
+                              int* rounds);
+/* stats[5] = rounds, bootstraps, gates, GPU-busy seconds, peak pool bytes in use. */
+int cufhe_server_stats(cufhe_server* server, double* stats);
+/* Finishes all submitted work, then frees the server. */
+int cufhe_server_destroy(cufhe_server* server);
+
 /* ---- components ----------------------------------------------------------
  * IdentityKeySwitch lvl1 -> lvl0 of n ciphertexts (reference
  * include/keyswitch_gpu.cuh:128-187 KeySwitchFromTLWE / SEIandKS
