@@ -168,7 +168,7 @@ def workload_kb4096(args):
     want = np.array([C.TRUTH[o](int(bits[i]), int(bits[B + i])) for i, o in enumerate(ops)],
                     np.uint8)
 
-    cands = [args.jobs_per_group] if args.jobs_per_group else [1, 2, 4, 8]
+    cands = [args.jobs_per_group] if args.jobs_per_group else [1, 2, 4, 6, 8]
     best_c, tune = autotune(C, gates, dpool, cands)
 
     plan = C.Plan(gates, dpool, jobs_per_group=best_c)
@@ -504,7 +504,8 @@ def main():
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="kb4096")
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--jobs-per-group", type=int, default=0,
-                    help="bootstraps per workgroup; 0 = autotune over {1, 2, 4, 8}")
+                    help="bootstraps per workgroup; 0 = autotune over {1, 2, 4, 6, 8} "
+                         "(6: key shared through LDS, FFT key bundle only)")
     ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--plain-streams", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)

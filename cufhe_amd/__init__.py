@@ -103,6 +103,11 @@ def lib():
         "cufhe_plan_run_timed": ([vp, vp, ctypes.POINTER(ctypes.c_float)], i32),
         "cufhe_gate_lvl1": ([i32, vp, vp, vp, vp, vp], i32),
         "cufhe_gate_lvl02": ([i32, vp, vp, vp, vp, vp], i32),
+        "cufhe_gate_lvl0": ([i32, vp, vp, vp, vp, vp], i32),
+        "cufhe_plan_create_lvl01": ([vp, i32, vp, i32, i32, ctypes.POINTER(vp)], i32),
+        "cufhe_sample_extract_key_switch_lvl10": ([vp, vp, i32, vp], i32),
+        "cufhe_bootstrap_to_trlwe_lvl01": ([vp, vp, u32, i32, vp], i32),
+        "cufhe_refresh_lvl1": ([vp, vp, i32, vp], i32),
         "cufhe_plan_create_lvl1": ([vp, i32, vp, i32, i32, ctypes.POINTER(vp)], i32),
         "cufhe_plan_run": ([vp, vp], i32),
         "cufhe_plan_create_lvl02": ([vp, i32, vp, i32, ctypes.POINTER(vp)], i32),
@@ -296,6 +301,9 @@ class Plan:
     """Batched gates over a device pool of lvl1 TLWEs."""
 
     def __init__(self, gates, pool, jobs_per_group=0, use_graph=False, level=1):
+        """level 1: lvl1 wires (IKS-BR); 2: lvl0 wires bootstrapped at lvl2
+        (BR-IKS, lvl02 keys); 0: lvl0 wires bootstrapped at lvl1 (BR-IKS,
+        lvl1 keys)."""
         arr = (GateDesc * len(gates))()
         for i, g in enumerate(gates):
             op, out, in0, in1, inc = g
@@ -306,6 +314,10 @@ class Plan:
             check(lib().cufhe_plan_create_lvl02(ctypes.cast(arr, ctypes.c_void_p), len(gates),
                                                 pool.ptr, int(use_graph), ctypes.byref(h)),
                   "cufhe_plan_create_lvl02")
+        elif level == 0:
+            check(lib().cufhe_plan_create_lvl01(ctypes.cast(arr, ctypes.c_void_p), len(gates),
+                                                pool.ptr, int(jobs_per_group), int(use_graph),
+                                                ctypes.byref(h)), "cufhe_plan_create_lvl01")
         else:
             check(lib().cufhe_plan_create_lvl1(ctypes.cast(arr, ctypes.c_void_p), len(gates),
                                                pool.ptr, int(jobs_per_group), int(use_graph),

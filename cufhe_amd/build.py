@@ -55,13 +55,15 @@ def build(jobs=4, verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
     os.replace(tmp, LIB)
-    # C++ drop-in gate test (tests/cpp), linked against the library in-tree
+    # C++ drop-in tests (tests/cpp/*.cpp), linked against the library in-tree
     root = os.path.dirname(HERE)
-    src = os.path.join(root, "tests", "cpp", "test_gate_gpu.cpp")
-    if os.path.exists(src):
-        exe = os.path.join(root, "tests", "cpp", "test_gate_gpu.bin")
-        _run([HIPCC, "-O2", "-std=c++17", "-I", os.path.join(root, "include"), src, "-o", exe,
-              "-L", HERE, "-lcufhe_amd", "-Wl,-rpath,$ORIGIN/../../cufhe_amd"])
+    cpp_dir = os.path.join(root, "tests", "cpp")
+    srcs = sorted(f for f in os.listdir(cpp_dir) if f.endswith(".cpp")) if os.path.isdir(cpp_dir) else []
+    cmds = [[HIPCC, "-O2", "-std=c++17", "-I", os.path.join(root, "include"),
+             os.path.join(cpp_dir, f), "-o", os.path.join(cpp_dir, f[:-4] + ".bin"),
+             "-L", HERE, "-lcufhe_amd", "-Wl,-rpath,$ORIGIN/../../cufhe_amd"] for f in srcs]
+    with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        list(ex.map(_run, cmds))
     return LIB
 
 

@@ -351,6 +351,17 @@ __device__ __forceinline__ void blind_rotate_fft(const uint32_t* tl0, typename P
     }
 }
 
+// The rotated TRLWE itself: component j's N coefficients at out + j N.
+template <class P>
+__device__ __forceinline__ void trlwe_store(const AccStore<P>& acc, int j, typename P::T* out)
+{
+    constexpr int N = P::n, R = N / 128;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int r = 0; r < 2 * R; r++)
+        out[j * N + AccStore<P>::coef_index(r, lane)] = acc.get(r, lane);
+}
+
 // SampleExtractIndex<0> (reference src/bootstrap_gpu.cu:1201-1221):
 // res[0] = a[0], res[i] = -a[N-i] (0 < i < N), res[N] = b[0].
 template <class P>

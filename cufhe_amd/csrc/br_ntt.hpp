@@ -253,6 +253,14 @@ __device__ __forceinline__ void blind_rotate_ntt(const uint32_t* tl0, uint32_t m
     }
 }
 
+// The rotated TRLWE itself: component j's N coefficients at out + j N.
+__device__ __forceinline__ void trlwe_store_ntt(const uint32_t (&acc)[16], int j, uint32_t* out)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int r = 0; r < 16; r++) out[j * lvl1param::n + ntt_coef_index(r, lane)] = acc[r];
+}
+
 // SampleExtractIndex<0> for the NTT coefficient layout.
 __device__ __forceinline__ void sample_extract_ntt(const uint32_t (&acc)[16], int j, uint32_t* out)
 {

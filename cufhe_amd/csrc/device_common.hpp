@@ -93,8 +93,12 @@ struct BrJob {
     int32_t ca;
     int32_t cb;
     uint32_t offset;
-    uint32_t pad;
+    uint32_t flags;  // kBrOutTrlwe: store the rotated TRLWE instead of its sample
 };
+// BrJob::flags bit: out receives the (k+1) N-word TRLWE (reference
+// BootstrapTLWE2TRLWE, src/bootstrap_gpu.cu:2371-2388) instead of the
+// extracted TLWE.
+constexpr uint32_t kBrOutTrlwe = 1;
 static_assert(sizeof(BrJob) == 40, "BrJob layout");
 
 // One key-switch job: out_lvl0 = KS(ca*in0 + cb*in1 (+offset on b)).

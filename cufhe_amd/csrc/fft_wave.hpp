@@ -14,8 +14,8 @@
 //   1. R-point DFT in registers over m2 (m = lane + 64*m2),
 //   2. twiddle W_H^(lane*k2),
 //   3. 64-point DFT over the lane index as 8x8 with two LDS transposes
-//      (contiguous / XOR-swizzled addressing, 16-byte accesses), no
-//      workgroup barrier: only the wave itself touches its scratch.
+//      (XOR-swizzled, bank-conflict-free 16-byte accesses), no workgroup
+//      barrier: only the wave itself touches its scratch.
 // Output: lane L = c + 8q, register g*8+d holds Z[k] for
 //   k = (q + 8g) + R*(c + 8d); it is stored at p = (g*8+d)*64 + L so that a
 //   wave reads a spectrum with fully coalesced 1 KiB accesses per register.
@@ -165,6 +165,26 @@ __device__ __forceinline__ void dft_reg(cplx* x)
     for (int i = 0; i < M; i++) x[i] = tmp[i];
 }
 
+// Transpose slots in the wave's scratch (16-byte elements). Rows are 64
+// elements. SWZ: a 16-lane group of a 128-bit LDS access must touch 16
+// distinct 4-bank units; odd rows are swizzled so that the two rows read or
+// written by the same 16 lanes (q = 2m, 2m+1) land in opposite bank halves
+// (conflict-free, at the price of per-lane address registers):
+//   step 1: element (row, col) at row*64 + (col ^ 8*(row&1))
+//   step 2: element (row, c, x) at row*64 + 8*(c ^ (row&1)) + x
+// Without SWZ the same slots without the XOR on the row parity (2-way
+// conflicts between the two rows of a 16-lane group).
+template <bool SWZ>
+__device__ __forceinline__ int fft_t1(int row, int col)
+{
+    return row * 64 + (SWZ ? (col ^ ((row & 1) << 3)) : col);
+}
+template <bool SWZ>
+__device__ __forceinline__ int fft_t2(int row, int c, int x)
+{
+    return row * 64 + ((SWZ ? (c ^ (row & 1)) : c) << 3) + x;
+}
+
 // Tables of the folded transform (device memory, built at init), H = 64 R:
 //   twl[k2*64 + l] = exp(i*pi*l/N) * exp(-2*pi*i*l*k2/H)   (lane twist merged
 //                    with the step-2 twiddle), k2 < R, l < 64
@@ -182,7 +202,7 @@ struct FFTTables {
 // (untwisted). Out: spectrum Z[k] = sum_m (p[m] + i p[m+H]) zeta_k^m with
 // zeta_k = exp(i*pi*(1-4k)/N), in the register layout described above.
 // lds: H complex scratch owned by this wave.
-template <int H>
+template <int H, bool SWZ = false>
 __device__ __forceinline__ void fft_forward(cplx (&x)[H / 64], cplx* lds, const FFTTables& tb)
 {
     constexpr int R = H / 64;
@@ -194,13 +214,13 @@ __device__ __forceinline__ void fft_forward(cplx (&x)[H / 64], cplx* lds, const 
 #pragma unroll
     for (int k2 = 0; k2 < R; k2++) x[k2] = cmul(x[k2], tb.twl[k2 * 64 + lane]);
 #pragma unroll
-    for (int k2 = 0; k2 < R; k2++) lds_store(&lds[k2 * 64 + lane], x[k2]);
+    for (int k2 = 0; k2 < R; k2++) lds_store(&lds[fft_t1<SWZ>(k2, lane)], x[k2]);
     wave_lds_sync();
     const int lo = lane & 7, q = lane >> 3;
 #pragma unroll
     for (int g = 0; g < G; g++)
 #pragma unroll
-        for (int b = 0; b < 8; b++) x[g * 8 + b] = lds_load(&lds[(g * 8 + q) * 64 + 8 * b + lo]);
+        for (int b = 0; b < 8; b++) x[g * 8 + b] = lds_load(&lds[fft_t1<SWZ>(g * 8 + q, 8 * b + lo)]);
     wave_lds_sync();
 #pragma unroll
     for (int g = 0; g < G; g++) {
@@ -208,15 +228,13 @@ __device__ __forceinline__ void fft_forward(cplx (&x)[H / 64], cplx* lds, const 
 #pragma unroll
         for (int c = 1; c < 8; c++) x[g * 8 + c] = cmul(x[g * 8 + c], tb.tw64[c * 8 + lo]);
 #pragma unroll
-        for (int c = 0; c < 8; c++)
-            lds_store(&lds[((g * 8 + q) * 8 + c) * 8 + (lo ^ c)], x[g * 8 + c]);
+        for (int c = 0; c < 8; c++) lds_store(&lds[fft_t2<SWZ>(g * 8 + q, c, lo ^ c)], x[g * 8 + c]);
     }
     wave_lds_sync();
 #pragma unroll
     for (int g = 0; g < G; g++)
 #pragma unroll
-        for (int a = 0; a < 8; a++)
-            x[g * 8 + a] = lds_load(&lds[((g * 8 + q) * 8 + lo) * 8 + (a ^ lo)]);
+        for (int a = 0; a < 8; a++) x[g * 8 + a] = lds_load(&lds[fft_t2<SWZ>(g * 8 + q, lo, a ^ lo)]);
     wave_lds_sync();
 #pragma unroll
     for (int g = 0; g < G; g++) dft_reg<8, false>(&x[g * 8]);
@@ -224,7 +242,7 @@ __device__ __forceinline__ void fft_forward(cplx (&x)[H / 64], cplx* lds, const 
 
 // Inverse folded transform without the 1/H factor: spectrum in the layout
 // fft_forward produces; result x[r] = H * (p[m] + i p[m+H]), m = lane + 64 r.
-template <int H>
+template <int H, bool SWZ = false>
 __device__ __forceinline__ void fft_inverse(cplx (&x)[H / 64], cplx* lds, const FFTTables& tb)
 {
     constexpr int R = H / 64;
@@ -235,15 +253,13 @@ __device__ __forceinline__ void fft_inverse(cplx (&x)[H / 64], cplx* lds, const 
     for (int g = 0; g < G; g++) {
         dft_reg<8, true>(&x[g * 8]);
 #pragma unroll
-        for (int a = 0; a < 8; a++)
-            lds_store(&lds[((g * 8 + q) * 8 + lo) * 8 + (a ^ lo)], x[g * 8 + a]);
+        for (int a = 0; a < 8; a++) lds_store(&lds[fft_t2<SWZ>(g * 8 + q, lo, a ^ lo)], x[g * 8 + a]);
     }
     wave_lds_sync();
 #pragma unroll
     for (int g = 0; g < G; g++)
 #pragma unroll
-        for (int c = 0; c < 8; c++)
-            x[g * 8 + c] = lds_load(&lds[((g * 8 + q) * 8 + c) * 8 + (lo ^ c)]);
+        for (int c = 0; c < 8; c++) x[g * 8 + c] = lds_load(&lds[fft_t2<SWZ>(g * 8 + q, c, lo ^ c)]);
     wave_lds_sync();
 #pragma unroll
     for (int g = 0; g < G; g++) {
@@ -252,11 +268,11 @@ __device__ __forceinline__ void fft_inverse(cplx (&x)[H / 64], cplx* lds, const 
         for (int c = 1; c < 8; c++) x[g * 8 + c] = cmulc(x[g * 8 + c], tb.tw64[c * 8 + lo]);
         dft_reg<8, true>(&x[g * 8]);
 #pragma unroll
-        for (int b = 0; b < 8; b++) lds_store(&lds[(g * 8 + q) * 64 + 8 * b + lo], x[g * 8 + b]);
+        for (int b = 0; b < 8; b++) lds_store(&lds[fft_t1<SWZ>(g * 8 + q, 8 * b + lo)], x[g * 8 + b]);
     }
     wave_lds_sync();
 #pragma unroll
-    for (int k2 = 0; k2 < R; k2++) x[k2] = lds_load(&lds[k2 * 64 + lane]);
+    for (int k2 = 0; k2 < R; k2++) x[k2] = lds_load(&lds[fft_t1<SWZ>(k2, lane)]);
     wave_lds_sync();
 #pragma unroll
     for (int k2 = 0; k2 < R; k2++) x[k2] = cmulc(x[k2], tb.twl[k2 * 64 + lane]);

@@ -1,6 +1,7 @@
 // FFT-backend kernels: key conversion, batched blind rotation, plain
 // per-gate kernels, key switching, MUX combine, and FFNT entry points.
 #include "br_fft.hpp"
+#include "br_fft_kbs.hpp"
 #include "keyswitch.hpp"
 #include "launch.hpp"
 
@@ -119,7 +120,128 @@ __global__ __launch_bounds__(128 * C, CUFHE_BR_WAVES_PER_SIMD) void k_br_fft(con
     AccStore<P> acc;
     if constexpr (kL) acc.lds = sh_coef[wave];
     blind_rotate_fft<P, KB>(sh_tl0[slot], mu, acc, j, sh_spec[wave], sh_spec[wave ^ 1], K);
-    if (live) sample_extract_store<P>(acc, j, static_cast<T*>(jb.out));
+    if (live) {
+        if (jb.flags & kBrOutTrlwe)
+            trlwe_store<P>(acc, j, static_cast<T*>(jb.out));
+        else
+            sample_extract_store<P>(acc, j, static_cast<T*>(jb.out));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched key-bundle rotation at lvl1 with the key shared by the workgroup's
+// C jobs through an LDS ring (br_fft_kbs.hpp). One workgroup of 2C waves per
+// CU (C = 6: 12 waves, 138 KiB LDS).
+template <int C>
+__global__ __launch_bounds__(128 * C, 1) void k_br_fft_kbs(const BrJob* __restrict__ jobs,
+                                                            int njobs, uint32_t mu,
+                                                            FourierKeys<lvl1param> K)
+{
+    using P = lvl1param;
+    constexpr int N = P::n, H = N / 2, R = H / 64, n0 = lvl0param::n;
+    constexpr int NT = 128 * C;
+    __shared__ KbsShared<C> sm;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int slot = wave >> 1, j = wave & 1;
+    const int t = threadIdx.x;
+    for (int i = t; i < R * 64; i += NT) sm.twl[i] = K.fft.twl[i];
+    for (int i = t; i < 64; i += NT) {
+        sm.tw64[i] = K.fft.tw64[i];
+        sm.e2n_lo[i] = K.e2n[i];
+    }
+    for (int i = t; i < R; i += NT) sm.cr[i] = K.fft.cr[i];
+    for (int i = t; i < 2 * N / 64; i += NT) sm.e2n_hi[i] = K.e2n[64 * i];
+    const int raw = blockIdx.x * C + slot;
+    const bool live = raw < njobs;
+    const BrJob jb = jobs[live ? raw : njobs - 1];
+    constexpr uint32_t rnd = kMaskRound<P>;
+    for (int i = t & 127; i <= n0; i += 128) {
+        uint32_t v = static_cast<uint32_t>(jb.ca) * jb.in0[i];
+        if (jb.in1) v += static_cast<uint32_t>(jb.cb) * jb.in1[i];
+        if (i < n0)
+            sm.abar[slot][i] = static_cast<uint16_t>(mod_switch<P>(v + rnd));
+        else
+            sm.bar[slot] = 2 * N - mod_switch<P>(v + jb.offset);
+    }
+    __syncthreads();
+    AccStore<P> acc;
+    blind_rotate_kbs<C>(sm, slot, j, mu, acc.reg, K.bk);
+    if (live) {
+        if (jb.flags & kBrOutTrlwe)
+            trlwe_store<P>(acc, j, static_cast<uint32_t*>(jb.out));
+        else
+            sample_extract_store<P>(acc, j, static_cast<uint32_t*>(jb.out));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain per-gate form, lvl0 API (BR-IKS at lvl1; reference
+// src/cufhe_gates_gpu.cu:198-511, src/bootstrap_gpu.cu:1557-1672): lvl0
+// inputs pre-added into the rotation, SampleExtract, key switch lvl1 -> lvl0.
+// MUX/NMUX rotate c + t - mu and -c + f - mu on two wave pairs and key-switch
+// +-(r1 + r0 + mu) in one pass.
+template <bool KB, bool MUX, bool SUBSET>
+__global__ __launch_bounds__(MUX ? 256 : 128, 2) void k_gate_lvl01_fft(
+    uint32_t* out, const uint32_t* in0, const uint32_t* in1, const uint32_t* inc, int32_t ca,
+    int32_t cb, uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
+    FourierKeys<lvl1param> K)
+{
+    using P = lvl1param;
+    constexpr int C = MUX ? 2 : 1;
+    constexpr int N = P::n, H = N / 2, n0 = lvl0param::n;
+    constexpr uint32_t mu = static_cast<uint32_t>(P::μ);
+    constexpr uint32_t mu0 = static_cast<uint32_t>(lvl0param::μ);
+    constexpr int NB = BrShape<P>::NB;
+    __shared__ cplx sh_spec[2 * C][NB * H];
+    __shared__ uint32_t sh_tl0[C][kTlwe0Pad];
+    __shared__ uint32_t sh_res[C][N + 4];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int slot = wave >> 1, j = wave & 1;
+    const int t = threadIdx.x & 127;
+    {
+        const uint32_t* a = MUX ? inc : in0;
+        const uint32_t* b = MUX ? (slot == 0 ? in1 : in0) : in1;
+        const int32_t sa = MUX ? (slot == 0 ? 1 : -1) : ca;
+        const int32_t sb = MUX ? 1 : cb;
+        const uint32_t off = MUX ? 0u - mu0 : offset;
+        for (int i = t; i <= n0; i += 128) {
+            uint32_t v = static_cast<uint32_t>(sa) * a[i] + static_cast<uint32_t>(sb) * b[i];
+            if (i == n0) v += off;
+            sh_tl0[slot][i] = v;
+        }
+    }
+    __syncthreads();
+    AccStore<P> acc;
+    blind_rotate_fft<P, KB>(sh_tl0[slot], mu, acc, j, sh_spec[wave], sh_spec[wave ^ 1], K);
+    sample_extract_store<P>(acc, j, sh_res[slot]);
+    __syncthreads();
+    if constexpr (MUX)
+        key_switch_columns<TFHEpp::lvl10param, SUBSET, 3>(out, sh_res[0], sh_res[1], mux_sign,
+                                                          mux_sign, mux_sign > 0 ? mu : 0u - mu,
+                                                          ksk, threadIdx.x, 256);
+    else
+        key_switch_columns<TFHEpp::lvl10param, SUBSET, 5>(out, sh_res[0], nullptr, 1, 0, 0u, ksk,
+                                                          t, 128);
+}
+
+// SampleExtract + key switch lvl1 -> lvl0 of n TRLWEs (reference SEIandKS,
+// src/keyswitch_gpu.cu:61-68, include/keyswitch_gpu.cuh:52-123), one job per
+// workgroup.
+template <bool SUBSET>
+__global__ __launch_bounds__(256) void k_se_ks_lvl10(uint32_t* out, const uint32_t* trlwe, int n,
+                                                     const uint32_t* ksk)
+{
+    constexpr int N = lvl1param::n, n0 = lvl0param::n;
+    __shared__ uint32_t tl[N + 4];
+    const int job = blockIdx.x;
+    if (job >= n) return;  // whole workgroup
+    const uint32_t* tr = trlwe + static_cast<size_t>(job) * 2 * N;
+    for (int i = threadIdx.x; i <= N; i += 256)
+        tl[i] = i == 0 ? tr[0] : (i < N ? 0u - tr[N - i] : tr[N]);
+    __syncthreads();
+    key_switch_columns<TFHEpp::lvl10param, SUBSET, 3>(out + static_cast<size_t>(job) * (n0 + 1),
+                                                      tl, nullptr, 1, 0, 0u, ksk, threadIdx.x,
+                                                      256);
 }
 
 // ---------------------------------------------------------------------------
@@ -343,6 +465,10 @@ void launch_br_fft(const BrJob* jobs, int njobs, int c, typename P::T mu,
                 hipLaunchKernelGGL((k_br_fft<P, KB, 8>), grid, block, 0, st, jobs, njobs, mu,
                                    K);
             break;
+        case 6:
+            if constexpr (KB && sizeof(typename P::T) == 4)
+                hipLaunchKernelGGL((k_br_fft_kbs<6>), grid, block, 0, st, jobs, njobs, mu, K);
+            break;
         default:
             break;
     }
@@ -372,6 +498,43 @@ void launch_gate_lvl1_fft(bool kb, bool mux, uint32_t* out, const uint32_t* in0,
         if (mux) CUFHE_L(false, true); else CUFHE_L(false, false);
     }
 #undef CUFHE_L
+}
+
+void launch_gate_lvl01_fft(bool kb, bool mux, bool subset, uint32_t* out, const uint32_t* in0,
+                           const uint32_t* in1, const uint32_t* inc, int32_t ca, int32_t cb,
+                           uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
+                           const FourierKeys<lvl1param>& K, hipStream_t st)
+{
+#define CUFHE_G01(KBV, MUXV, SUB)                                                              \
+    hipLaunchKernelGGL((k_gate_lvl01_fft<KBV, MUXV, SUB>), dim3(1), dim3(MUXV ? 256 : 128), 0, \
+                       st, out, in0, in1, inc, ca, cb, offset, mux_sign, ksk, K)
+    if (kb) {
+        if (mux) {
+            if (subset) CUFHE_G01(true, true, true); else CUFHE_G01(true, true, false);
+        }
+        else {
+            if (subset) CUFHE_G01(true, false, true); else CUFHE_G01(true, false, false);
+        }
+    }
+    else {
+        if (mux) {
+            if (subset) CUFHE_G01(false, true, true); else CUFHE_G01(false, true, false);
+        }
+        else {
+            if (subset) CUFHE_G01(false, false, true); else CUFHE_G01(false, false, false);
+        }
+    }
+#undef CUFHE_G01
+}
+
+void launch_se_ks_lvl10(bool subset, uint32_t* out, const uint32_t* trlwe, int n,
+                        const uint32_t* ksk, hipStream_t st)
+{
+    if (n <= 0) return;
+    if (subset)
+        hipLaunchKernelGGL((k_se_ks_lvl10<true>), dim3(n), dim3(256), 0, st, out, trlwe, n, ksk);
+    else
+        hipLaunchKernelGGL((k_se_ks_lvl10<false>), dim3(n), dim3(256), 0, st, out, trlwe, n, ksk);
 }
 
 void launch_gate_lvl02_fft(bool kb, bool mux, uint32_t* out, const uint32_t* in0,

@@ -97,7 +97,56 @@ __global__ __launch_bounds__(128 * C, CUFHE_BR_NTT_WAVES_PER_SIMD) void k_br_ntt
     __syncthreads();
     uint32_t acc[16];
     blind_rotate_ntt<MODE>(sh_tl0[slot], mu, acc, j, sh_lds[wave], sh_lds[wave ^ 1], K);
-    if (live) sample_extract_ntt(acc, j, static_cast<uint32_t*>(jb.out));
+    if (live) {
+        if (jb.flags & kBrOutTrlwe)
+            trlwe_store_ntt(acc, j, static_cast<uint32_t*>(jb.out));
+        else
+            sample_extract_ntt(acc, j, static_cast<uint32_t*>(jb.out));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain per-gate form, lvl0 API (BR-IKS at lvl1), as k_gate_lvl01_fft.
+template <int MODE, bool SUBSET, bool MUX>
+__global__ __launch_bounds__(MUX ? 256 : 128, 2) void k_gate_lvl01_ntt(
+    uint32_t* out, const uint32_t* in0, const uint32_t* in1, const uint32_t* inc, int32_t ca,
+    int32_t cb, uint32_t offset, int32_t mux_sign, const uint32_t* ksk, NttKeys K)
+{
+    using P = lvl1param;
+    constexpr int C = MUX ? 2 : 1;
+    constexpr int N = P::n, n0 = lvl0param::n;
+    constexpr uint32_t mu = static_cast<uint32_t>(P::μ);
+    constexpr uint32_t mu0 = static_cast<uint32_t>(lvl0param::μ);
+    __shared__ uint32_t sh_lds[2 * C][kNttLdsWords];
+    __shared__ uint32_t sh_tl0[C][kNttTlwe0Pad];
+    __shared__ uint32_t sh_res[C][N + 4];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int slot = wave >> 1, j = wave & 1;
+    const int t = threadIdx.x & 127;
+    {
+        const uint32_t* a = MUX ? inc : in0;
+        const uint32_t* b = MUX ? (slot == 0 ? in1 : in0) : in1;
+        const int32_t sa = MUX ? (slot == 0 ? 1 : -1) : ca;
+        const int32_t sb = MUX ? 1 : cb;
+        const uint32_t off = MUX ? 0u - mu0 : offset;
+        for (int i = t; i <= n0; i += 128) {
+            uint32_t v = static_cast<uint32_t>(sa) * a[i] + static_cast<uint32_t>(sb) * b[i];
+            if (i == n0) v += off;
+            sh_tl0[slot][i] = v;
+        }
+    }
+    __syncthreads();
+    uint32_t acc[16];
+    blind_rotate_ntt<MODE>(sh_tl0[slot], mu, acc, j, sh_lds[wave], sh_lds[wave ^ 1], K);
+    sample_extract_ntt(acc, j, sh_res[slot]);
+    __syncthreads();
+    if constexpr (MUX)
+        key_switch_columns<TFHEpp::lvl10param, SUBSET, 3>(out, sh_res[0], sh_res[1], mux_sign,
+                                                          mux_sign, mux_sign > 0 ? mu : 0u - mu,
+                                                          ksk, threadIdx.x, 256);
+    else
+        key_switch_columns<TFHEpp::lvl10param, SUBSET, 5>(out, sh_res[0], nullptr, 1, 0, 0u, ksk,
+                                                          t, 128);
 }
 
 // ---------------------------------------------------------------------------
@@ -230,6 +279,34 @@ void launch_gate_lvl1_ntt(int mode, bool subset, bool mux, uint32_t* out, const 
     if (mode == NTT_BB) { CUFHE_GATE_NTT(NTT_BB, false); }
     CUFHE_GATE_NTT(NTT_PLAIN, false);
 #undef CUFHE_GATE_NTT
+}
+
+void launch_gate_lvl01_ntt(int mode, bool subset, bool mux, uint32_t* out, const uint32_t* in0,
+                           const uint32_t* in1, const uint32_t* inc, int32_t ca, int32_t cb,
+                           uint32_t offset, int32_t mux_sign, const uint32_t* ksk, const NttKeys& K,
+                           hipStream_t st)
+{
+#define CUFHE_G01N(M, S, X)                                                                     \
+    hipLaunchKernelGGL((k_gate_lvl01_ntt<M, S, X>), dim3(1), dim3(X ? 256 : 128), 0, st, out, in0, \
+                       in1, inc, ca, cb, offset, mux_sign, ksk, K)
+#define CUFHE_G01N_MODE(M)                                                \
+    if (subset) {                                                         \
+        if (mux) CUFHE_G01N(M, true, true); else CUFHE_G01N(M, true, false); \
+    }                                                                     \
+    else {                                                                \
+        if (mux) CUFHE_G01N(M, false, true); else CUFHE_G01N(M, false, false); \
+    }
+    if (mode == NTT_KB) {
+        CUFHE_G01N_MODE(NTT_KB)
+    }
+    else if (mode == NTT_BB) {
+        CUFHE_G01N_MODE(NTT_BB)
+    }
+    else {
+        CUFHE_G01N_MODE(NTT_PLAIN)
+    }
+#undef CUFHE_G01N_MODE
+#undef CUFHE_G01N
 }
 
 }  // namespace cufhe_amd

@@ -30,6 +30,12 @@ void launch_gate_lvl1_fft(bool kb, bool mux, uint32_t* out, const uint32_t* in0,
                           const uint32_t* in1, const uint32_t* inc, int32_t ca, int32_t cb,
                           uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
                           const FourierKeys<lvl1param>& K, hipStream_t st);
+void launch_gate_lvl01_fft(bool kb, bool mux, bool subset, uint32_t* out, const uint32_t* in0,
+                           const uint32_t* in1, const uint32_t* inc, int32_t ca, int32_t cb,
+                           uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
+                           const FourierKeys<lvl1param>& K, hipStream_t st);
+void launch_se_ks_lvl10(bool subset, uint32_t* out, const uint32_t* trlwe, int n,
+                        const uint32_t* ksk, hipStream_t st);
 void launch_gate_lvl02_fft(bool kb, bool mux, uint32_t* out, const uint32_t* in0,
                            const uint32_t* in1, const uint32_t* inc, int32_t ca, int32_t cb,
                            uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
@@ -49,6 +55,10 @@ void launch_ntt_natural(bool inverse, uint32_t* out, const uint32_t* in, int npo
                         const NttTables& tb, hipStream_t st);
 void launch_br_ntt(int mode, const BrJob* jobs, int njobs, int c, uint32_t mu, const NttKeys& K,
                    hipStream_t st);
+void launch_gate_lvl01_ntt(int mode, bool subset, bool mux, uint32_t* out, const uint32_t* in0,
+                           const uint32_t* in1, const uint32_t* inc, int32_t ca, int32_t cb,
+                           uint32_t offset, int32_t mux_sign, const uint32_t* ksk, const NttKeys& K,
+                           hipStream_t st);
 void launch_gate_lvl1_ntt(int mode, bool subset, bool mux, uint32_t* out, const uint32_t* in0,
                           const uint32_t* in1, const uint32_t* inc, int32_t ca, int32_t cb,
                           uint32_t offset, int32_t mux_sign, const uint32_t* ksk, const NttKeys& K,

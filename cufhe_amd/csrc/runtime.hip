@@ -97,7 +97,16 @@ std::vector<cplx> e2n_table(int N)
     e[N / 2] = {0.0, 1.0};
     e[N] = {-1.0, 0.0};
     e[3 * N / 2] = {0.0, -1.0};
-    return e;
+    // Store each root as the FP64 product of e[64 (t >> 6)] and e[t & 63],
+    // rounded exactly as the device's cmul rounds it: kernels that rebuild
+    // e[t] from those two short tables (in LDS) and kernels that gather e[t]
+    // then see identical values. Entries t < 64 and t = 64u are unchanged.
+    std::vector<cplx> p(2 * N);
+    for (int t = 0; t < 2 * N; t++) {
+        const cplx a = e[64 * (t >> 6)], b = e[t & 63];
+        p[t] = cplx{std::fma(a.x, b.x, -a.y * b.y), std::fma(a.x, b.y, a.y * b.x)};
+    }
+    return p;
 }
 
 template <class P>
@@ -469,6 +478,9 @@ int lvl1_pick_c(int c)
     // default from the MI355X sweep of {1, 2, 4, 8} (lvl1, 4096 gates):
     // one job per workgroup wins once the key stream is L2-resident
     if (c == 0) c = 1;
+    // 6: the key-bundle FFT kernel that shares key chunks through LDS
+    const LevelKeys<lvl1param>& k = state().k1;
+    if (c == 6) return (k.backend == CUFHE_BACKEND_FFT && k.kb) ? 6 : -1;
     if (c != 1 && c != 2 && c != 4 && c != 8) return -1;
     return c;
 }
@@ -486,7 +498,7 @@ void lvl1_launch(const KsJob* ks, int nks, const BrJob* br, int nbr, const LinJo
 using namespace cufhe_amd;
 
 struct cufhe_plan {
-    int level = 1;  // 1: lvl1 gates (IKS-BR); 2: lvl02 gates (BR-IKS)
+    int level = 1;  // 1: lvl1 gates (IKS-BR); 2: lvl02 gates (BR-IKS); 3: lvl0 gates (BR-IKS at lvl1)
     int njobs_ks = 0, njobs_br = 0, njobs_lin = 0;
     int c = 2;
     bool use_graph = false;
@@ -504,6 +516,12 @@ namespace {
 int plan_launch(cufhe_plan* p, hipStream_t st)
 {
     State& s = state();
+    if (p->level == 3) {
+        br_lvl1(p->br, p->njobs_br, p->c, static_cast<uint32_t>(lvl1param::μ), st);
+        ks_lvl1(p->ks, p->njobs_ks, st);
+        if (p->njobs_lin) launch_linear(p->lin, p->njobs_lin, st);
+        return check_launch("batched lvl0 gate launch");
+    }
     if (p->level == 2) {
         // BR-IKS order: rotations (pre-added lvl0 inputs) -> lvl2 TLWEs ->
         // key switch (MUX halves combined in its pre-add)
@@ -792,7 +810,9 @@ int cufhe_plan_create_lvl1(const cufhe_gate_desc* gates, int n, uint32_t* d_pool
     int rc = lvl1_count(gates, n, &nks, &nmux);
     if (rc != CUFHE_OK) return rc;
     const int c = pick_c(jobs_per_group, s.k1.kb, nks);
-    if (c < 0) return fail(CUFHE_ERR_UNSUPPORTED, "jobs_per_group must be 0, 1, 2, 4 or 8");
+    if (c < 0)
+        return fail(CUFHE_ERR_UNSUPPORTED,
+                    "jobs_per_group must be 0, 1, 2, 4 or 8 (6: FFT key-bundle keys only)");
     cufhe_plan* p = new cufhe_plan();
     p->c = c;
     p->use_graph = use_graph != 0;
@@ -819,6 +839,164 @@ int cufhe_plan_create_lvl1(const cufhe_gate_desc* gates, int n, uint32_t* d_pool
     p->njobs_lin = static_cast<int>(jobs.lin.size());
     *out = p;
     return CUFHE_OK;
+}
+
+int cufhe_plan_create_lvl01(const cufhe_gate_desc* gates, int n, uint32_t* d_pool,
+                            int jobs_per_group, int use_graph, cufhe_plan** out)
+{
+    State& s = state();
+    if (!out) return fail(CUFHE_ERR_ARG, "null plan handle");
+    *out = nullptr;
+    if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded (cufhe_initialize_lvl01)");
+    if (n < 0 || (n > 0 && (!gates || !d_pool))) return fail(CUFHE_ERR_ARG, "bad gate list");
+    constexpr uint32_t W0 = lvl0param::n + 1;  // lvl0 wire stride
+    constexpr uint32_t W1 = lvl1param::n + 1;  // rotated lvl1 TLWE in the workspace
+    constexpr uint32_t mu0 = static_cast<uint32_t>(lvl0param::μ);
+    constexpr uint32_t mu1 = static_cast<uint32_t>(lvl1param::μ);
+    int nbr = 0, nmux = 0;
+    int rc = lvl1_count(gates, n, &nbr, &nmux);
+    if (rc != CUFHE_OK) return rc;
+    const int c = pick_c(jobs_per_group, s.k1.kb, nbr);
+    if (c < 0)
+        return fail(CUFHE_ERR_UNSUPPORTED,
+                    "jobs_per_group must be 0, 1, 2, 4 or 8 (6: FFT key-bundle keys only)");
+    cufhe_plan* p = new cufhe_plan();
+    p->level = 3;
+    p->c = c;
+    p->use_graph = use_graph != 0;
+    if (nbr) {
+        if (hipMalloc(reinterpret_cast<void**>(&p->ws),
+                      static_cast<size_t>(nbr) * W1 * sizeof(uint32_t)) != hipSuccess) {
+            delete p;
+            return fail(CUFHE_ERR_NOMEM, "plan workspace allocation failed");
+        }
+    }
+    std::vector<KsJob> ks;
+    std::vector<BrJob> br;
+    std::vector<LinJob> lin;
+    auto wire = [&](uint32_t w) { return d_pool + static_cast<size_t>(w) * W0; };
+    for (int i = 0; i < n; i++) {
+        const cufhe_gate_desc& gd = gates[i];
+        if (gd.op == CUFHE_NOT || gd.op == CUFHE_COPY) {
+            lin.push_back(LinJob{wire(gd.in0), nullptr, wire(gd.out), gd.op == CUFHE_NOT ? -1 : 1, 0,
+                                 0u, static_cast<int32_t>(W0)});
+            continue;
+        }
+        if (gd.op == CUFHE_MUX || gd.op == CUFHE_NMUX) {
+            uint32_t* r1 = p->ws + br.size() * W1;
+            br.push_back(BrJob{wire(gd.inc), wire(gd.in1), r1, 1, 1, 0u - mu0, 0u});
+            uint32_t* r0 = p->ws + br.size() * W1;
+            br.push_back(BrJob{wire(gd.inc), wire(gd.in0), r0, -1, 1, 0u - mu0, 0u});
+            const int sg = gd.op == CUFHE_MUX ? 1 : -1;
+            ks.push_back(KsJob{r1, r0, wire(gd.out), sg, sg,
+                               static_cast<uint64_t>(sg > 0 ? mu1 : 0u - mu1)});
+            continue;
+        }
+        GateLin gl;
+        gate_lin(gd.op, gl);
+        uint32_t* r = p->ws + br.size() * W1;
+        br.push_back(BrJob{wire(gd.in0), wire(gd.in1), r, gl.ca, gl.cb,
+                           static_cast<uint32_t>(gl.off_mu) * mu0, 0u});
+        ks.push_back(KsJob{r, nullptr, wire(gd.out), 1, 0, 0});
+    }
+    rc = alloc_upload(&p->ks, ks);
+    if (rc == CUFHE_OK) rc = alloc_upload(&p->br, br);
+    if (rc == CUFHE_OK) rc = alloc_upload(&p->lin, lin);
+    if (rc != CUFHE_OK) {
+        cufhe_plan_destroy(p);
+        return rc;
+    }
+    p->njobs_ks = static_cast<int>(ks.size());
+    p->njobs_br = static_cast<int>(br.size());
+    p->njobs_lin = static_cast<int>(lin.size());
+    *out = p;
+    return CUFHE_OK;
+}
+
+int cufhe_gate_lvl0(int op, uint32_t* out, const uint32_t* in0, const uint32_t* in1,
+                    const uint32_t* inc, void* stream)
+{
+    State& s = state();
+    if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded (cufhe_initialize_lvl01)");
+    if (!out || !in0) return fail(CUFHE_ERR_ARG, "null ciphertext pointer");
+    const hipStream_t st = as_stream(stream);
+    constexpr uint32_t mu0 = static_cast<uint32_t>(lvl0param::μ);
+    const LevelKeys<lvl1param>& k = s.k1;
+    if (op == CUFHE_NOT || op == CUFHE_COPY) {
+        launch_linear_one(LinJob{in0, nullptr, out, op == CUFHE_NOT ? -1 : 1, 0, 0u,
+                                 static_cast<int32_t>(lvl0param::n + 1)},
+                          st);
+        return check_launch("not/copy lvl0");
+    }
+    const bool mux = op == CUFHE_MUX || op == CUFHE_NMUX;
+    GateLin gl{0, 0, 0};
+    if (mux) {
+        if (!in1 || !inc) return fail(CUFHE_ERR_ARG, "null MUX operand");
+    }
+    else {
+        if (!gate_lin(op, gl)) return fail(CUFHE_ERR_ARG, "unknown gate op " + std::to_string(op));
+        if (!in1) return fail(CUFHE_ERR_ARG, "null second operand");
+    }
+    const uint32_t off = static_cast<uint32_t>(gl.off_mu) * mu0;
+    const int sign = op == CUFHE_NMUX ? -1 : 1;
+    if (k.backend == CUFHE_BACKEND_NTT)
+        launch_gate_lvl01_ntt(k.variant, k.subset, mux, out, in0, in1, inc, gl.ca, gl.cb, off, sign,
+                              k.ksk, k.nk, st);
+    else
+        launch_gate_lvl01_fft(k.kb, mux, k.subset, out, in0, in1, inc, gl.ca, gl.cb, off, sign, k.ksk,
+                              k.fk, st);
+    return check_launch("lvl0 gate");
+}
+
+int cufhe_sample_extract_key_switch_lvl10(uint32_t* d_out, const uint32_t* d_trlwe, int n,
+                                          void* stream)
+{
+    State& s = state();
+    if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded");
+    if (n <= 0) return n == 0 ? CUFHE_OK : fail(CUFHE_ERR_ARG, "negative count");
+    if (!d_out || !d_trlwe) return fail(CUFHE_ERR_ARG, "null pointer");
+    launch_se_ks_lvl10(s.k1.subset, d_out, d_trlwe, n, s.k1.ksk, as_stream(stream));
+    return check_launch("sample extract + key switch");
+}
+
+int cufhe_bootstrap_to_trlwe_lvl01(uint32_t* d_out, const uint32_t* d_in, uint32_t mu, int n,
+                                   void* stream)
+{
+    State& s = state();
+    if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded");
+    if (n <= 0) return n == 0 ? CUFHE_OK : fail(CUFHE_ERR_ARG, "negative count");
+    if (!d_out || !d_in) return fail(CUFHE_ERR_ARG, "null pointer");
+    const hipStream_t st = as_stream(stream);
+    std::vector<BrJob> jobs(n);
+    for (int i = 0; i < n; i++)
+        jobs[i] = BrJob{d_in + static_cast<size_t>(i) * (lvl0param::n + 1), nullptr,
+                        d_out + static_cast<size_t>(i) * 2 * lvl1param::n, 1, 0, 0u, kBrOutTrlwe};
+    BrJob* d = nullptr;
+    CUFHE_HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&d), n * sizeof(BrJob), st));
+    CUFHE_HIP_TRY(hipMemcpyAsync(d, jobs.data(), n * sizeof(BrJob), hipMemcpyHostToDevice, st));
+    br_lvl1(d, n, 1, mu, st);
+    CUFHE_HIP_TRY(hipFreeAsync(d, st));
+    CUFHE_HIP_TRY(hipStreamSynchronize(st));
+    return check_launch("bootstrap to TRLWE");
+}
+
+int cufhe_refresh_lvl1(uint32_t* d_out, const uint32_t* d_in, int n, void* stream)
+{
+    State& s = state();
+    if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded");
+    if (n <= 0) return n == 0 ? CUFHE_OK : fail(CUFHE_ERR_ARG, "negative count");
+    if (!d_out || !d_in) return fail(CUFHE_ERR_ARG, "null pointer");
+    const hipStream_t st = as_stream(stream);
+    uint32_t* tl0 = nullptr;
+    CUFHE_HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&tl0),
+                                 static_cast<size_t>(n) * (lvl0param::n + 1) * sizeof(uint32_t), st));
+    launch_se_ks_lvl10(s.k1.subset, tl0, d_in, n, s.k1.ksk, st);
+    int rc = check_launch("refresh key switch");
+    if (rc == CUFHE_OK)
+        rc = cufhe_bootstrap_to_trlwe_lvl01(d_out, tl0, static_cast<uint32_t>(lvl1param::μ), n, stream);
+    (void)hipFreeAsync(tl0, st);
+    (void)hipStreamSynchronize(st);
+    return rc;
 }
 
 int cufhe_plan_create_lvl02(const cufhe_gate_desc* gates, int n, uint32_t* d_pool,
@@ -901,6 +1079,8 @@ int cufhe_plan_run(cufhe_plan* p, void* stream)
     State& s = state();
     if (!(p->level == 2 ? s.k2.loaded : s.k1.loaded))
         return fail(CUFHE_ERR_STATE, "keys of the plan's level not loaded");
+    if (p->level != 2 && p->c != lvl1_pick_c(p->c))
+        return fail(CUFHE_ERR_STATE, "jobs_per_group of the plan does not fit the loaded lvl1 keys");
     const hipStream_t st = as_stream(stream);
     if (!p->use_graph) return plan_launch(p, st);
     if (!p->exec) {
@@ -976,12 +1156,22 @@ int cufhe_plan_run_timed(cufhe_plan* p, void* stream, float* kernel_ms)
         return rc2;
     }
     if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded");
+    if (p->c != lvl1_pick_c(p->c))
+        return fail(CUFHE_ERR_STATE, "jobs_per_group of the plan does not fit the loaded lvl1 keys");
     hipEvent_t ev[4];
     for (auto& e : ev) CUFHE_HIP_TRY(hipEventCreate(&e));
     CUFHE_HIP_TRY(hipEventRecord(ev[0], st));
-    ks_lvl1(p->ks, p->njobs_ks, st);
-    CUFHE_HIP_TRY(hipEventRecord(ev[1], st));
-    br_lvl1(p->br, p->njobs_br, p->c, static_cast<uint32_t>(lvl1param::μ), st);
+    if (p->level == 3) {
+        // BR-IKS order; reported as [0] key switch, [1] rotation, [2] combine
+        br_lvl1(p->br, p->njobs_br, p->c, static_cast<uint32_t>(lvl1param::μ), st);
+        CUFHE_HIP_TRY(hipEventRecord(ev[1], st));
+        ks_lvl1(p->ks, p->njobs_ks, st);
+    }
+    else {
+        ks_lvl1(p->ks, p->njobs_ks, st);
+        CUFHE_HIP_TRY(hipEventRecord(ev[1], st));
+        br_lvl1(p->br, p->njobs_br, p->c, static_cast<uint32_t>(lvl1param::μ), st);
+    }
     CUFHE_HIP_TRY(hipEventRecord(ev[2], st));
     if (p->njobs_lin) launch_linear(p->lin, p->njobs_lin, st);
     CUFHE_HIP_TRY(hipEventRecord(ev[3], st));
@@ -990,8 +1180,10 @@ int cufhe_plan_run_timed(cufhe_plan* p, void* stream, float* kernel_ms)
         const hipError_t e = hipEventSynchronize(ev[3]);
         if (e != hipSuccess) rc = fail(CUFHE_ERR_HIP, std::string("timed plan: ") + hipGetErrorString(e));
     }
-    if (rc == CUFHE_OK)
+    if (rc == CUFHE_OK) {
         for (int i = 0; i < 3; i++) (void)hipEventElapsedTime(&kernel_ms[i], ev[i], ev[i + 1]);
+        if (p->level == 3) std::swap(kernel_ms[0], kernel_ms[1]);
+    }
     for (auto& e : ev) (void)hipEventDestroy(e);
     return rc;
 }

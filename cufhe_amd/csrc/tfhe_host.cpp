@@ -384,6 +384,16 @@ int tfhe_tlwe_encrypt(int level, uint64_t seed, uint64_t index, const void* key,
     }
 }
 
+int tfhe_trlwe_encrypt(uint64_t seed, uint64_t index, const uint32_t* key, const uint32_t* m,
+                       double alpha, uint32_t* ct)
+{
+    if (!key || !m || !ct) return CUFHE_ERR_ARG;
+    ChaCha20 rng(seed, kTagEnc | index);
+    trlwe_encrypt_zero<lvl1param>(ct, key, alpha, rng);
+    for (uint32_t i = 0; i < lvl1param::n; i++) ct[lvl1param::n + i] += m[i];
+    return CUFHE_OK;
+}
+
 int tfhe_tlwe_phase(int level, const void* key, const void* ct, size_t count, double* out)
 {
     if (!key || (!ct && count) || (!out && count)) return CUFHE_ERR_ARG;

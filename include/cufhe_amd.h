@@ -138,6 +138,12 @@ int cufhe_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
  * out may alias any input. */
 int cufhe_gate_lvl1(int op, uint32_t* out, const uint32_t* in0, const uint32_t* in1,
                     const uint32_t* inc, void* stream);
+/* lvl0 API with the lvl1 keys, BR-IKS order (reference gNand<lvl0param> ...
+ * gNMux<lvl0param>, src/cufhe_gates_gpu.cu:198-511 -> bootstrap_gpu.cu
+ * :1557-1672): rotation of the pre-added lvl0 inputs at lvl1, SampleExtract,
+ * key switch lvl1 -> lvl0 (the subset key when loaded so). 637-word TLWEs. */
+int cufhe_gate_lvl0(int op, uint32_t* out, const uint32_t* in0, const uint32_t* in1,
+                    const uint32_t* inc, void* stream);
 /* lvl02 API, BR-IKS order: lvl0 TLWEs in and out, bootstrapped at lvl2
  * (reference Nand_lvl02 ... NMux_lvl02, src/cufhe_gates_gpu.cu:826-977). */
 int cufhe_gate_lvl02(int op, uint32_t* out, const uint32_t* in0, const uint32_t* in1,
@@ -148,8 +154,9 @@ int cufhe_gate_lvl02(int op, uint32_t* out, const uint32_t* in0, const uint32_t*
 
 /* Plan a batch of n gates over a device pool of lvl1 TLWEs (wire w at
  * d_pool + 1025 w). Outputs of a batch must not be read by the same batch.
- * jobs_per_group: bootstraps sharing one key pass per workgroup (1, 2, 4;
- * 0 = autotuned choice). use_graph != 0 captures the launches in a HIP
+ * jobs_per_group: bootstraps per workgroup (1, 2, 4, 8; 6 = the FFT
+ * key-bundle kernel that shares each key chunk among the workgroup's six
+ * jobs through LDS; 0 = default). use_graph != 0 captures the launches in a HIP
  * graph on the first run and replays it. */
 int cufhe_plan_create_lvl1(const cufhe_gate_desc* gates, int n, uint32_t* d_pool,
                            int jobs_per_group, int use_graph, cufhe_plan** plan);
@@ -210,6 +217,11 @@ int cufhe_server_stats(cufhe_server* server, double* stats);
 /* Finishes all submitted work, then frees the server. */
 int cufhe_server_destroy(cufhe_server* server);
 
+/* The same for lvl0 wires with the lvl1 keys (BR-IKS at lvl1, as
+ * cufhe_gate_lvl0): wire w at d_pool + 637 w. */
+int cufhe_plan_create_lvl01(const cufhe_gate_desc* gates, int n, uint32_t* d_pool,
+                            int jobs_per_group, int use_graph, cufhe_plan** plan);
+
 /* ---- components ----------------------------------------------------------
  * IdentityKeySwitch lvl1 -> lvl0 of n ciphertexts (reference
  * include/keyswitch_gpu.cuh:128-187 KeySwitchFromTLWE / SEIandKS
@@ -220,6 +232,18 @@ int cufhe_identity_key_switch_lvl10(uint32_t* d_out, const uint32_t* d_in, int n
  * in: n x 637, out: n x 1025 words. */
 int cufhe_bootstrap_lvl01(uint32_t* d_out, const uint32_t* d_in, uint32_t mu, int n,
                           int jobs_per_group, void* stream);
+/* TRLWE extras (reference src/cufhe_gates_gpu.cu:134-196): n lvl1 TRLWEs
+ * (2 x 1024 words: a then b) -> SampleExtract + key switch -> n lvl0 TLWEs
+ * (SampleExtractAndKeySwitch / SEIandKS, src/keyswitch_gpu.cu:61-68); n lvl0
+ * TLWEs -> rotated test-vector TRLWEs without extraction
+ * (GateBootstrappingTLWE2TRLWElvl01NTT / BootstrapTLWE2TRLWE,
+ * src/bootstrap_gpu.cu:2371-2388); TRLWE -> SE + key switch -> rotation ->
+ * TRLWE (Refresh / SEIandBootstrap2TRLWE, :2390-2414). */
+int cufhe_sample_extract_key_switch_lvl10(uint32_t* d_out, const uint32_t* d_trlwe, int n,
+                                          void* stream);
+int cufhe_bootstrap_to_trlwe_lvl01(uint32_t* d_out, const uint32_t* d_in, uint32_t mu, int n,
+                                   void* stream);
+int cufhe_refresh_lvl1(uint32_t* d_out, const uint32_t* d_in, int n, void* stream);
 /* The same at lvl2 with the lvl02 keys (reference src/bootstrap_gpu.cu
  * :1248-1273, the rotation + SampleExtract half of the BR-IKS gate): in
  * n x 637 uint32 lvl0 TLWEs, out n x 2049 uint64 lvl2 TLWEs. */

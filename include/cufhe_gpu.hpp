@@ -6,7 +6,8 @@
 // CuSafeCall does (include/error_gpu.cuh:30-66).
 //
 // Differences a caller can see: one GPU per process (SetGPUNum(n > 1) is
-// refused), and lvl0-input gates (the BR-IKS lvl01 form) are not provided.
+// refused); the backend (FFT / NTT block-binary) follows the EvalKey instead
+// of a build option.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -46,8 +47,14 @@ inline void Initialize(const TFHEpp::EvalKey& ek)
         std::fprintf(stderr, "cufhe: Initialize(ek) needs bklvl01 and iksklvl10\n");
         std::exit(-1);
     }
-    CuCheck(cufhe_initialize_lvl01(ek.bklvl01->data(), ek.bklvl01->size(), ek.key_bundle,
-                                   ek.iksklvl10->data(), ek.iksklvl10->size()),
+    // reference build options as key properties: block-binary keys run on the
+    // NTT backend (USE_BLOCK_BINARY implies USE_FFT=OFF), others on the FFT one
+    const int backend = ek.block_binary ? CUFHE_BACKEND_NTT : CUFHE_BACKEND_FFT;
+    const int variant = ek.block_binary ? CUFHE_BR_BLOCK_BINARY
+                                        : (ek.key_bundle ? CUFHE_BR_KEY_BUNDLE : CUFHE_BR_PLAIN);
+    CuCheck(cufhe_initialize_lvl01_ex(ek.bklvl01->data(), ek.bklvl01->size(), backend, variant,
+                                      ek.iksklvl10->data(), ek.iksklvl10->size(),
+                                      ek.subset_iksk ? 1 : 0),
             "Initialize");
 }
 
@@ -118,6 +125,22 @@ inline bool StreamQuery(Stream st)
     return r == 1;
 }
 
+/** lvl1 TRLWE with one device copy (reference cufhe_gpu.cuh:123-160). */
+struct cuFHETRLWElvl1 {
+    cuFHETRLWElvl1()
+    {
+        void* p = nullptr;
+        CuCheck(cufhe_device_alloc(&p, sizeof(trlwehost)), "TRLWE alloc");
+        trlwedevices.assign(_gpuNum, static_cast<uint32_t*>(p));
+    }
+    ~cuFHETRLWElvl1() { cufhe_device_free(trlwedevices[0]); }
+    cuFHETRLWElvl1(const cuFHETRLWElvl1&) = delete;
+    cuFHETRLWElvl1& operator=(const cuFHETRLWElvl1&) = delete;
+
+    TFHEpp::TRLWE<TFHEpp::lvl1param> trlwehost;  // (a, b), 2 x 1024 words
+    std::vector<uint32_t*> trlwedevices;
+};
+
 template <class P>
 inline void CtxtCopyH2D(Ctxt<P>& c, Stream st)
 {
@@ -139,14 +162,16 @@ void CopyOnHost(Ctxt<P>& out, Ctxt<P>& in)
 }
 
 namespace detail {
+// lvl1 ciphertexts: IKS-BR gates; lvl0 ciphertexts: BR-IKS gates at lvl1
+// (reference src/cufhe_gates_gpu.cu:198-511 and :514-825).
 template <class P>
 inline void gate(int op, Ctxt<P>& out, Ctxt<P>* in0, Ctxt<P>* in1, Ctxt<P>* inc, Stream st)
 {
-    static_assert(std::is_same_v<P, TFHEpp::lvl1param>,
-                  "the lvl1 gate API takes Ctxt<lvl1param>; lvl0 gates are the *_lvl02 functions");
-    CuCheck(cufhe_gate_lvl1(op, out.tlwedevices[0], in0 ? in0->tlwedevices[0] : nullptr,
-                            in1 ? in1->tlwedevices[0] : nullptr, inc ? inc->tlwedevices[0] : nullptr,
-                            st.st()),
+    static_assert(std::is_same_v<P, TFHEpp::lvl1param> || std::is_same_v<P, TFHEpp::lvl0param>,
+                  "gates take Ctxt<lvl0param> or Ctxt<lvl1param>");
+    auto* fn = std::is_same_v<P, TFHEpp::lvl1param> ? cufhe_gate_lvl1 : cufhe_gate_lvl0;
+    CuCheck(fn(op, out.tlwedevices[0], in0 ? in0->tlwedevices[0] : nullptr,
+               in1 ? in1->tlwedevices[0] : nullptr, inc ? inc->tlwedevices[0] : nullptr, st.st()),
             "gate");
 }
 template <class P>
@@ -252,6 +277,54 @@ template <class P>
 void gCopy(Ctxt<P>& out, Ctxt<P>& in, Stream st)
 {
     detail::gate<P>(CUFHE_COPY, out, &in, nullptr, nullptr, st);
+}
+
+// TRLWE extras (reference src/cufhe_gates_gpu.cu:134-196).
+inline void gSampleExtractAndKeySwitch(Ctxt<TFHEpp::lvl0param>& out, const cuFHETRLWElvl1& in,
+                                       Stream st)
+{
+    CuCheck(cufhe_sample_extract_key_switch_lvl10(out.tlwedevices[0], in.trlwedevices[0], 1,
+                                                  st.st()),
+            "SampleExtractAndKeySwitch");
+}
+inline void SampleExtractAndKeySwitch(Ctxt<TFHEpp::lvl0param>& out, const cuFHETRLWElvl1& in,
+                                      Stream st)
+{
+    CuCheck(cufhe_memcpy_h2d(in.trlwedevices[0], in.trlwehost.data(), sizeof(in.trlwehost),
+                             st.st()),
+            "TRLWE H2D");
+    gSampleExtractAndKeySwitch(out, in, st);
+    CtxtCopyD2H<TFHEpp::lvl0param>(out, st);
+}
+inline void gGateBootstrappingTLWE2TRLWElvl01NTT(cuFHETRLWElvl1& out,
+                                                 Ctxt<TFHEpp::lvl0param>& in, Stream st)
+{
+    CuCheck(cufhe_bootstrap_to_trlwe_lvl01(out.trlwedevices[0], in.tlwedevices[0],
+                                           TFHEpp::lvl1param::μ, 1, st.st()),
+            "GateBootstrappingTLWE2TRLWElvl01NTT");
+}
+inline void GateBootstrappingTLWE2TRLWElvl01NTT(cuFHETRLWElvl1& out,
+                                                Ctxt<TFHEpp::lvl0param>& in, Stream st)
+{
+    CtxtCopyH2D<TFHEpp::lvl0param>(in, st);
+    gGateBootstrappingTLWE2TRLWElvl01NTT(out, in, st);
+    CuCheck(cufhe_memcpy_d2h(out.trlwehost.data(), out.trlwedevices[0], sizeof(out.trlwehost),
+                             st.st()),
+            "TRLWE D2H");
+}
+inline void gRefresh(cuFHETRLWElvl1& out, cuFHETRLWElvl1& in, Stream st)
+{
+    CuCheck(cufhe_refresh_lvl1(out.trlwedevices[0], in.trlwedevices[0], 1, st.st()), "Refresh");
+}
+inline void Refresh(cuFHETRLWElvl1& out, cuFHETRLWElvl1& in, Stream st)
+{
+    CuCheck(cufhe_memcpy_h2d(in.trlwedevices[0], in.trlwehost.data(), sizeof(in.trlwehost),
+                             st.st()),
+            "TRLWE H2D");
+    gRefresh(out, in, st);
+    CuCheck(cufhe_memcpy_d2h(out.trlwehost.data(), out.trlwedevices[0], sizeof(out.trlwehost),
+                             st.st()),
+            "TRLWE D2H");
 }
 
 }  // namespace cufhe

@@ -60,6 +60,11 @@ int tfhe_tlwe_encrypt_bits(int level, uint64_t seed, const void* key,
 int tfhe_tlwe_encrypt(int level, uint64_t seed, uint64_t index, const void* key, uint64_t m,
                       double alpha, void* ct_out);
 
+/* One lvl1 TRLWE encryption (a then b, 2 x 1024 words) of the Torus32
+ * polynomial m under the lvl1 key (TFHEpp trlweSymEncrypt). */
+int tfhe_trlwe_encrypt(uint64_t seed, uint64_t index, const uint32_t* key_lvl1, const uint32_t* m,
+                       double alpha, uint32_t* ct_out);
+
 /* Phase of count TLWEs as a torus fraction in [-1/2, 1/2). */
 int tfhe_tlwe_phase(int level, const void* key, const void* ct, size_t count,
                     double* phase_out);

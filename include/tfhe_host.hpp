@@ -60,9 +60,11 @@ struct SecretKey {
         }
     } key;
     uint64_t seed;
-    explicit SecretKey(uint64_t s) : seed(s)
+    bool block_binary = false;  // lvl0 key in blocks of lvl0bbparam::ell, lvl1 key starts with it
+    explicit SecretKey(uint64_t s, bool bb = false) : seed(s), block_binary(bb)
     {
-        detail::check(tfhe_secret_key_gen(s, 0, key.lvl0.data(), key.lvl1.data(), key.lvl2.data()),
+        detail::check(tfhe_secret_key_gen(s, bb ? 1 : 0, key.lvl0.data(), key.lvl1.data(),
+                                          key.lvl2.data()),
                       "tfhe_secret_key_gen");
     }
     SecretKey() : SecretKey(std::random_device{}() | (static_cast<uint64_t>(std::random_device{}()) << 32)) {}
@@ -70,7 +72,19 @@ struct SecretKey {
 
 // Bootstrapping keys (polynomial domain) and key-switching keys.
 struct EvalKey {
-    bool key_bundle = true;  // 2-bit key bundle (reference default build)
+    bool key_bundle = true;    // 2-bit key bundle (reference default build)
+    bool block_binary = false;  // block-binary rotation (NTT backend; reference USE_BLOCK_BINARY)
+    bool subset_iksk = false;   // iksklvl10 holds the subset key (emplacesubiksk)
+    EvalKey() = default;
+    // Keys for sk's configuration: a block-binary secret key selects the
+    // block-binary rotation (one TRGSW per key bit, no key bundle).
+    explicit EvalKey(const SecretKey& sk)
+    {
+        if (sk.block_binary) {
+            key_bundle = false;
+            block_binary = true;
+        }
+    }
     std::unique_ptr<std::vector<uint32_t>> bklvl01;
     std::unique_ptr<std::vector<uint64_t>> bklvl02;
     std::unique_ptr<std::vector<uint32_t>> iksklvl10;
@@ -92,6 +106,19 @@ struct EvalKey {
                                       sk.key.lvl2.data(), bklvl02->data(), bklvl02->size(), nullptr),
                           "tfhe_bk_gen");
         }
+    }
+    // Subset key switching lvl1 -> lvl0 (reference emplacesubiksk,
+    // test/test_block_binary_gpu.cc:155): rows only for lvl1 coefficients
+    // beyond the lvl0 key the lvl1 key starts with.
+    template <class P>
+    void emplacesubiksk(const SecretKey& sk)
+    {
+        static_assert(std::is_same_v<P, lvl10param>, "subset key switching is lvl10 only");
+        iksklvl10 = std::make_unique<std::vector<uint32_t>>(tfhe_ksk_subset_words());
+        detail::check(tfhe_ksk_gen(1, sk.seed ^ 0x510, sk.key.lvl1.data(), sk.key.lvl0.data(), 1,
+                                   iksklvl10->data(), iksklvl10->size()),
+                      "tfhe_ksk_gen");
+        subset_iksk = true;
     }
     template <class P>
     void emplaceiksk(const SecretKey& sk)
@@ -121,6 +148,32 @@ TLWE<P> tlweSymEncrypt(const typename P::T p, const double alpha, const Key<P>& 
                                     alpha, c.data()),
                   "tfhe_tlwe_encrypt");
     return c;
+}
+
+inline void trlweSymEncrypt(TRLWE<lvl1param>& c, const Polynomial<lvl1param>& p,
+                            const Key<lvl1param>& key)
+{
+    std::array<uint32_t, 2 * lvl1param::n> flat;
+    detail::check(tfhe_trlwe_encrypt(detail::encrypt_seed(), detail::encrypt_counter()++,
+                                     key.data(), p.data(), lvl1param::α, flat.data()),
+                  "tfhe_trlwe_encrypt");
+    for (uint32_t i = 0; i < lvl1param::n; i++) {
+        c[0][i] = flat[i];
+        c[1][i] = flat[lvl1param::n + i];
+    }
+}
+template <class P>
+void trlweSymEncrypt(TRLWE<P>& c, const Polynomial<P>& p, const Key<P>& key)
+{
+    static_assert(std::is_same_v<P, lvl1param>, "lvl1 TRLWE only");
+    trlweSymEncrypt(c, p, key);
+}
+
+// TFHEpp's in-place form: tlweSymEncrypt<P>(c, p, key) with the level's noise.
+template <class P>
+void tlweSymEncrypt(TLWE<P>& c, const typename P::T p, const Key<P>& key)
+{
+    c = tlweSymEncrypt<P>(p, P::α, key);
 }
 
 template <class P>

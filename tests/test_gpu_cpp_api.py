@@ -11,10 +11,23 @@ from conftest import ROOT
 pytestmark = pytest.mark.gpu
 
 
-def test_cpp_gate_api_truth_tables():
-    exe = os.path.join(ROOT, "tests", "cpp", "test_gate_gpu.bin")
+def _run(name):
+    exe = os.path.join(ROOT, "tests", "cpp", name)
     assert os.path.exists(exe), "build with `python cufhe_amd/build.py` first"
     p = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
                        timeout=240)
     print(p.stdout)
+    return p
+
+
+def test_cpp_gate_api_truth_tables():
+    p = _run("test_gate_gpu.bin")
     assert p.returncode == 0 and "PASS" in p.stdout
+
+
+def test_cpp_block_binary_api():
+    """Block-binary keys + subset key switching through the C++ drop-in:
+    lvl0 and lvl1 NAND/MUX truth tables and subset SampleExtractAndKeySwitch
+    (reference test/test_block_binary_gpu.cc)."""
+    p = _run("test_block_binary_gpu.bin")
+    assert p.returncode == 0 and "ALL PASS" in p.stdout
