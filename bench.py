@@ -355,7 +355,7 @@ def workload_nttbb(args):
     gates = [(ops[i], 2 * B + i, i, B + i, 0) for i in range(B)]
     want = np.array([C.TRUTH[o](int(bits[i]), int(bits[B + i])) for i, o in enumerate(ops)],
                     np.uint8)
-    cands = [args.jobs_per_group] if args.jobs_per_group else [1, 2, 4]
+    cands = [args.jobs_per_group] if args.jobs_per_group else [1, 2, 4, 6]
     best_c, tune = autotune(C, gates, dpool, cands)
     plan = C.Plan(gates, dpool, jobs_per_group=best_c)
     nbr = plan.bootstraps

@@ -2,6 +2,7 @@
 // rotation (plain CMUX, key bundle, block-binary), the plain per-gate form,
 // and transform entry points for the parity tests.
 #include "br_ntt.hpp"
+#include "br_ntt_shared.hpp"
 #include "keyswitch.hpp"
 #include "launch.hpp"
 
@@ -97,6 +98,77 @@ __global__ __launch_bounds__(128 * C, CUFHE_BR_NTT_WAVES_PER_SIMD) void k_br_ntt
     __syncthreads();
     uint32_t acc[16];
     blind_rotate_ntt<MODE>(sh_tl0[slot], mu, acc, j, sh_lds[wave], sh_lds[wave ^ 1], K);
+    if (live) {
+        if (jb.flags & kBrOutTrlwe)
+            trlwe_store_ntt(acc, j, static_cast<uint32_t*>(jb.out));
+        else
+            sample_extract_ntt(acc, j, static_cast<uint32_t*>(jb.out));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched rotation with the key shared by the workgroup's C jobs through an
+// LDS ring (br_ntt_shared.hpp); one workgroup of 2C waves per CU.
+template <int MODE, int C>
+__global__ __launch_bounds__(128 * C, 1) void k_br_ntt_shared(const BrJob* __restrict__ jobs,
+                                                               int njobs, uint32_t mu, NttKeys K)
+{
+    using P = lvl1param;
+    constexpr int N = P::n, n0 = lvl0param::n;
+    constexpr int NT = 128 * C;
+    constexpr uint32_t rnd = kMaskRound<P>;
+    constexpr int shift = 32 - 1 - P::nbit;
+    __shared__ NttShared<C> sm;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int slot = wave >> 1, j = wave & 1;
+    const int t = threadIdx.x;
+    for (int i = t; i < 16 * 64; i += NT) {
+        sm.twl[i] = K.tb.twl[i];
+        sm.twli[i] = K.tb.twli[i];
+    }
+    for (int i = t; i < 64; i += NT) {
+        sm.tw64[i] = K.tb.tw64[i];
+        sm.tw64i[i] = K.tb.tw64i[i];
+    }
+    for (int i = t; i < 16; i += NT) {
+        sm.cr[i] = K.tb.cr[i];
+        sm.cri[i] = K.tb.cri[i];
+    }
+    for (int i = t; i < 2 * N; i += NT) sm.pw[i] = K.tb.pw[i];
+    const int raw = blockIdx.x * C + slot;
+    const bool live = raw < njobs;
+    const BrJob jb = jobs[live ? raw : njobs - 1];
+    uint32_t part = 0;
+    for (int i = t & 127; i <= n0; i += 128) {
+        uint32_t v = static_cast<uint32_t>(jb.ca) * jb.in0[i];
+        if (jb.in1) v += static_cast<uint32_t>(jb.cb) * jb.in1[i];
+        if (i < n0) {
+            const uint32_t m = mod_switch<P>(v + rnd);
+            sm.abar[slot][i] = static_cast<uint16_t>(m);
+            part += v - (m << shift);
+        }
+        else {
+            sm.bar[slot] = v + jb.offset;  // body; turned into bar below
+        }
+    }
+    if constexpr (MODE == NTT_BB) {
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) part += __shfl_xor(part, m);
+        if ((t & 63) == 0) sm.corr[slot][j] = part;
+    }
+    __syncthreads();
+    if ((t & 127) == 0) {
+        uint32_t body = sm.bar[slot];
+        if constexpr (MODE == NTT_BB) {
+            // reference gatebootstrapping_gpu.cuh:814-838
+            const int32_t correction = static_cast<int32_t>(sm.corr[slot][0] + sm.corr[slot][1]);
+            body = body - static_cast<uint32_t>(correction / 2) + rnd;
+        }
+        sm.bar[slot] = 2 * N - mod_switch<P>(body);
+    }
+    __syncthreads();
+    uint32_t acc[16];
+    blind_rotate_ntt_shared<MODE, C>(sm, slot, j, mu, acc, K);
     if (live) {
         if (jb.flags & kBrOutTrlwe)
             trlwe_store_ntt(acc, j, static_cast<uint32_t*>(jb.out));
@@ -229,6 +301,10 @@ static void launch_br_ntt_mode(const BrJob* jobs, int njobs, int c, uint32_t mu,
         case 8:
             hipLaunchKernelGGL((k_br_ntt<MODE, 8>), dim3((njobs + 7) / 8), dim3(1024), 0, st, jobs,
                                njobs, mu, K);
+            break;
+        case 6:
+            hipLaunchKernelGGL((k_br_ntt_shared<MODE, 6>), dim3((njobs + 5) / 6), dim3(768), 0, st,
+                               jobs, njobs, mu, K);
             break;
         default:
             hipLaunchKernelGGL((k_br_ntt<MODE, 1>), dim3(njobs), dim3(128), 0, st, jobs, njobs, mu,

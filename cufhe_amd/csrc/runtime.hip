@@ -478,9 +478,10 @@ int lvl1_pick_c(int c)
     // default from the MI355X sweep of {1, 2, 4, 8} (lvl1, 4096 gates):
     // one job per workgroup wins once the key stream is L2-resident
     if (c == 0) c = 1;
-    // 6: the key-bundle FFT kernel that shares key chunks through LDS
+    // 6: the kernels that share key chunks among six jobs through LDS (NTT
+    // backend, all modes; FFT backend, key bundle)
     const LevelKeys<lvl1param>& k = state().k1;
-    if (c == 6) return (k.backend == CUFHE_BACKEND_FFT && k.kb) ? 6 : -1;
+    if (c == 6) return (k.backend == CUFHE_BACKEND_NTT || k.kb) ? 6 : -1;
     if (c != 1 && c != 2 && c != 4 && c != 8) return -1;
     return c;
 }

@@ -103,7 +103,7 @@ VARIANTS = ["plain", "kb", "bb"]
 
 
 @pytest.mark.parametrize("name", VARIANTS)
-@pytest.mark.parametrize("jpg", [1, 2])
+@pytest.mark.parametrize("jpg", [1, 2, 6])
 def test_bootstrap_bit_exact(engine, keys_lvl1, keys_bb, name, jpg):
     bk, _, okb, _ = load(name, keys_lvl1, keys_bb)
     rng = np.random.default_rng(60 + jpg)
@@ -169,14 +169,18 @@ def test_block_binary_batch_4096_nand_xor(engine, keys_lvl1, keys_bb):
     pool_ct = C.encrypt_bits(1, 777, k1, bits)
     pool = np.zeros((3 * n, W1), np.uint32)
     pool[: 2 * n] = pool_ct
-    dpool = C.DeviceBuffer.from_numpy(pool)
     gates = [(str(ops[i]), 2 * n + i, i, n + i, 0) for i in range(n)]
-    plan = C.Plan(gates, dpool, jobs_per_group=0, use_graph=True)
-    for _ in range(2):
-        plan.run()
-        C.check(C.lib().cufhe_synchronize(), "sync")
-    out = dpool.download(np.uint32, pool.shape)[2 * n:]
-    plan.destroy()
+    res = []
+    for c in (1, 6):  # per-job key loads / key shared by six jobs through LDS
+        dpool = C.DeviceBuffer.from_numpy(pool)
+        plan = C.Plan(gates, dpool, jobs_per_group=c, use_graph=True)
+        for _ in range(2):
+            plan.run()
+            C.check(C.lib().cufhe_synchronize(), "sync")
+        res.append(dpool.download(np.uint32, pool.shape)[2 * n:])
+        plan.destroy()
+    assert np.array_equal(res[0], res[1])
+    out = res[1]
     want = np.array([C.TRUTH[str(o)](int(bits[i]), int(bits[n + i])) for i, o in enumerate(ops)],
                     np.uint8)
     dec = C.decrypt_bits(1, k1, out)
