@@ -475,12 +475,15 @@ bool lvl1_loaded() { return state().k1.loaded; }
 
 int lvl1_pick_c(int c)
 {
-    // default from the MI355X sweep of {1, 2, 4, 8} (lvl1, 4096 gates):
-    // one job per workgroup wins once the key stream is L2-resident
-    if (c == 0) c = 1;
+    const LevelKeys<lvl1param>& k = state().k1;
+    // defaults from the MI355X sweeps (lvl1, 4096 gates, BR ms per launch):
+    //   FFT KB : C=1 24.0, 2 24.5, 4 34.2, 6 (shared key) 18.1, 8 38.9
+    //   NTT BB : C=1 18.1, 2 17.3, 4 20.3, 6 (shared key) 21.6
+    // the 16-byte FFT key stream is what sharing pays for; the 4-byte NTT
+    // key is not worth the extra barriers
+    if (c == 0) c = k.backend == CUFHE_BACKEND_NTT ? 2 : (k.kb ? 6 : 1);
     // 6: the kernels that share key chunks among six jobs through LDS (NTT
     // backend, all modes; FFT backend, key bundle)
-    const LevelKeys<lvl1param>& k = state().k1;
     if (c == 6) return (k.backend == CUFHE_BACKEND_NTT || k.kb) ? 6 : -1;
     if (c != 1 && c != 2 && c != 4 && c != 8) return -1;
     return c;
