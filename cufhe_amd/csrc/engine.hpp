@@ -39,8 +39,9 @@ inline size_t lvl1_workspace_words(int nks, int nmux)
 // jobs, using ws (lvl1_workspace_words of this list) for intermediates.
 void lvl1_build(const cufhe_gate_desc* gates, int n, uint32_t* d_pool, uint32_t* ws, Lvl1Jobs& jobs);
 bool lvl1_loaded();
-// Validated jobs-per-workgroup (0 = default), or -1.
-int lvl1_pick_c(int c);
+// Validated jobs-per-workgroup for nbr bootstraps (c = 0: the default for
+// the loaded keys and that batch size; nbr < 0: size unknown), or -1.
+int lvl1_pick_c(int c, int nbr);
 // Key switch -> blind rotation + SampleExtract -> combine, on stream st.
 void lvl1_launch(const KsJob* ks, int nks, const BrJob* br, int nbr, const LinJob* lin, int nlin,
                  int c, hipStream_t st);

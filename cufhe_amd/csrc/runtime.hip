@@ -473,9 +473,13 @@ void lvl1_build(const cufhe_gate_desc* gates, int n, uint32_t* d_pool, uint32_t*
 
 bool lvl1_loaded() { return state().k1.loaded; }
 
-int lvl1_pick_c(int c)
+int lvl1_pick_c(int c, int nbr)
 {
     const LevelKeys<lvl1param>& k = state().k1;
+    // a shared-key workgroup holds a whole CU: below ~6 jobs per CU the
+    // one-job workgroups spread over more CUs and finish sooner
+    constexpr int kSharedMinJobs = 6 * 256;
+    if (c == 0 && nbr >= 0 && nbr < kSharedMinJobs) c = 1;
     // defaults from the MI355X sweeps (lvl1, 4096 gates, BR ms per launch):
     //   FFT KB : C=1 24.0, 2 24.5, 4 34.2, 6 (shared key) 18.1, 8 38.9
     //   NTT BB : C=1 18.1, 2 17.3, 4 20.3, 6 (shared key) 21.6
@@ -548,8 +552,7 @@ int plan_launch(cufhe_plan* p, hipStream_t st)
 int pick_c(int c, bool kb, int nbr)
 {
     (void)kb;
-    (void)nbr;
-    return lvl1_pick_c(c);
+    return lvl1_pick_c(c, nbr);
 }
 
 }  // namespace
@@ -1083,7 +1086,7 @@ int cufhe_plan_run(cufhe_plan* p, void* stream)
     State& s = state();
     if (!(p->level == 2 ? s.k2.loaded : s.k1.loaded))
         return fail(CUFHE_ERR_STATE, "keys of the plan's level not loaded");
-    if (p->level != 2 && p->c != lvl1_pick_c(p->c))
+    if (p->level != 2 && p->c != lvl1_pick_c(p->c, -1))
         return fail(CUFHE_ERR_STATE, "jobs_per_group of the plan does not fit the loaded lvl1 keys");
     const hipStream_t st = as_stream(stream);
     if (!p->use_graph) return plan_launch(p, st);
@@ -1160,7 +1163,7 @@ int cufhe_plan_run_timed(cufhe_plan* p, void* stream, float* kernel_ms)
         return rc2;
     }
     if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded");
-    if (p->c != lvl1_pick_c(p->c))
+    if (p->c != lvl1_pick_c(p->c, -1))
         return fail(CUFHE_ERR_STATE, "jobs_per_group of the plan does not fit the loaded lvl1 keys");
     hipEvent_t ev[4];
     for (auto& e : ev) CUFHE_HIP_TRY(hipEventCreate(&e));

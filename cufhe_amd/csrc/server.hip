@@ -216,7 +216,9 @@ void worker_loop(cufhe_server* S)
         if (e == hipSuccess && nlin)
             e = hipMemcpyAsync(S->d_lin, S->h_lin, nlin * sizeof(LinJob), hipMemcpyHostToDevice, S->st);
         if (e == hipSuccess) {
-            lvl1_launch(S->d_ks, nks, S->d_br, nbr, S->d_lin, nlin, S->c, S->st);
+            // c = 0: choose per round (small rounds use one job per workgroup)
+            const int c = S->c ? S->c : lvl1_pick_c(0, nbr);
+            lvl1_launch(S->d_ks, nks, S->d_br, nbr, S->d_lin, nlin, c, S->st);
             e = hipGetLastError();
         }
         for (auto& r : finishing) {
@@ -269,7 +271,7 @@ int cufhe_server_create(size_t pool_bytes, int max_batch_bootstraps, int jobs_pe
     if (!server) return set_error(CUFHE_ERR_ARG, "null server handle");
     *server = nullptr;
     if (!lvl1_loaded()) return set_error(CUFHE_ERR_STATE, "lvl1 keys not loaded (cufhe_initialize_lvl01)");
-    const int c = lvl1_pick_c(jobs_per_group);
+    const int c = jobs_per_group == 0 ? 0 : lvl1_pick_c(jobs_per_group, -1);
     if (c < 0) return set_error(CUFHE_ERR_UNSUPPORTED, "jobs_per_group must be 0, 1, 2, 4 or 8");
     if (max_batch_bootstraps <= 0) max_batch_bootstraps = 8192;
     const size_t wire_bytes = static_cast<size_t>(kWireWords1) * sizeof(uint32_t);
