@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcufhe_amd.so")
 
-HIP_SOURCES = ["kernels_fft.hip", "kernels_ntt.hip", "runtime.hip", "server.hip"]
+HIP_SOURCES = ["kernels_fft.hip", "kernels_ntt.hip", "ks_gemm.hip", "runtime.hip", "server.hip"]
 CXX_SOURCES = ["tfhe_host.cpp"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -53,7 +53,8 @@ def build(jobs=4, verbose=False):
             if verbose and out.strip():
                 print(out)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs +
+         ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     # C++ drop-in tests (tests/cpp/*.cpp), linked against the library in-tree
     root = os.path.dirname(HERE)

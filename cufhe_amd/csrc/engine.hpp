@@ -42,8 +42,12 @@ bool lvl1_loaded();
 // Validated jobs-per-workgroup for nbr bootstraps (c = 0: the default for
 // the loaded keys and that batch size; nbr < 0: size unknown), or -1.
 int lvl1_pick_c(int c, int nbr);
-// Key switch -> blind rotation + SampleExtract -> combine, on stream st.
-void lvl1_launch(const KsJob* ks, int nks, const BrJob* br, int nbr, const LinJob* lin, int nlin,
-                 int c, hipStream_t st);
+// Workspace bytes of the GEMM key switch for nks jobs (0: gather path).
+size_t lvl1_ks_workspace_bytes(int nks);
+// Key switch -> blind rotation + SampleExtract -> combine, on stream st
+// (ks_ws: lvl1_ks_workspace_bytes(nks) bytes or null). False if the key
+// switch failed to launch.
+bool lvl1_launch(const KsJob* ks, int nks, void* ks_ws, const BrJob* br, int nbr,
+                 const LinJob* lin, int nlin, int c, hipStream_t st);
 
 }  // namespace cufhe_amd

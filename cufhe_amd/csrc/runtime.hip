@@ -15,6 +15,7 @@
 #include "br_fft.hpp"
 #include "br_ntt.hpp"
 #include "engine.hpp"
+#include "ks_gemm.hpp"
 #include "launch.hpp"
 
 namespace cufhe_amd {
@@ -62,6 +63,7 @@ struct LevelKeys {
     cplx* bk = nullptr;
     uint32_t* bk_ntt = nullptr;
     uint32_t* ksk = nullptr;
+    KsGemmKey ksg;  // byte planes of ksk for the GEMM key switch
     FourierKeys<P> fk{};
     NttKeys nk{};
 };
@@ -221,6 +223,7 @@ void free_keys(LevelKeys<P>& k)
     (void)hipFree(k.bk);
     (void)hipFree(k.bk_ntt);
     (void)hipFree(k.ksk);
+    ks_gemm_release(k.ksg);
     k = LevelKeys<P>{};
 }
 
@@ -383,10 +386,24 @@ void br_lvl1(const BrJob* jobs, int n, int c, uint32_t mu, hipStream_t st)
 }
 
 // Batched lvl1 -> lvl0 identity key switch (full or subset key).
-void ks_lvl1(const KsJob* jobs, int n, hipStream_t st)
+// ws: GEMM workspace for n jobs (ks_gemm_workspace_bytes) or null. Returns
+// false only if the GEMM path failed to launch.
+bool ks_lvl1(const KsJob* jobs, int n, void* ws, hipStream_t st)
 {
     const LevelKeys<lvl1param>& k = state().k1;
-    if (n > 0) launch_ks_batch(1, k.subset, jobs, n, k.ksk, st);
+    if (n <= 0) return true;
+    if (ws && n >= kKsGemmMinJobs && k.ksg.planes) return ks_gemm_run(k.ksg, jobs, n, ws, st) == 0;
+    launch_ks_batch(1, k.subset, jobs, n, k.ksk, st);
+    return true;
+}
+
+bool ks_lvl2(const KsJob* jobs, int n, void* ws, hipStream_t st)
+{
+    const LevelKeys<lvl2param>& k = state().k2;
+    if (n <= 0) return true;
+    if (ws && n >= kKsGemmMinJobs && k.ksg.planes) return ks_gemm_run(k.ksg, jobs, n, ws, st) == 0;
+    launch_ks_batch(2, false, jobs, n, k.ksk, st);
+    return true;
 }
 
 // One lvl1 gate (IKS-BR) as a single launch on the loaded backend.
@@ -493,12 +510,19 @@ int lvl1_pick_c(int c, int nbr)
     return c;
 }
 
-void lvl1_launch(const KsJob* ks, int nks, const BrJob* br, int nbr, const LinJob* lin, int nlin,
-                 int c, hipStream_t st)
+size_t lvl1_ks_workspace_bytes(int nks)
 {
-    ks_lvl1(ks, nks, st);
+    const LevelKeys<lvl1param>& k = state().k1;
+    return (nks >= kKsGemmMinJobs && k.ksg.planes) ? ks_gemm_workspace_bytes(k.ksg, nks) : 0;
+}
+
+bool lvl1_launch(const KsJob* ks, int nks, void* ks_ws, const BrJob* br, int nbr,
+                 const LinJob* lin, int nlin, int c, hipStream_t st)
+{
+    const bool ok = ks_lvl1(ks, nks, ks_ws, st);
     br_lvl1(br, nbr, c, static_cast<uint32_t>(lvl1param::μ), st);
     if (nlin) launch_linear(lin, nlin, st);
+    return ok;
 }
 
 }  // namespace cufhe_amd
@@ -514,6 +538,7 @@ struct cufhe_plan {
     BrJob* br = nullptr;
     LinJob* lin = nullptr;
     uint32_t* ws = nullptr;  // lvl0 TLWEs + MUX partial results
+    void* ks_ws = nullptr;   // GEMM key-switch workspace (large batches)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;
@@ -521,12 +546,24 @@ struct cufhe_plan {
 
 namespace {
 
+// GEMM key-switch workspace of a plan (batches of kKsGemmMinJobs or more).
+int plan_alloc_ks_ws(cufhe_plan* p)
+{
+    State& s = state();
+    const KsGemmKey& key = p->level == 2 ? s.k2.ksg : s.k1.ksg;
+    if (p->njobs_ks < kKsGemmMinJobs || !key.planes) return CUFHE_OK;
+    if (hipMalloc(&p->ks_ws, ks_gemm_workspace_bytes(key, p->njobs_ks)) != hipSuccess)
+        return fail(CUFHE_ERR_NOMEM, "key-switch workspace allocation failed");
+    return CUFHE_OK;
+}
+
 int plan_launch(cufhe_plan* p, hipStream_t st)
 {
     State& s = state();
     if (p->level == 3) {
         br_lvl1(p->br, p->njobs_br, p->c, static_cast<uint32_t>(lvl1param::μ), st);
-        ks_lvl1(p->ks, p->njobs_ks, st);
+        if (!ks_lvl1(p->ks, p->njobs_ks, p->ks_ws, st))
+            return fail(CUFHE_ERR_HIP, "GEMM key switch failed");
         if (p->njobs_lin) launch_linear(p->lin, p->njobs_lin, st);
         return check_launch("batched lvl0 gate launch");
     }
@@ -541,11 +578,13 @@ int plan_launch(cufhe_plan* p, hipStream_t st)
                 launch_br_fft<lvl2param, false>(p->br, p->njobs_br, 1,
                                                 static_cast<uint64_t>(lvl2param::μ), s.k2.fk, st);
         }
-        if (p->njobs_ks) launch_ks_batch(2, false, p->ks, p->njobs_ks, s.k2.ksk, st);
+        if (!ks_lvl2(p->ks, p->njobs_ks, p->ks_ws, st))
+            return fail(CUFHE_ERR_HIP, "GEMM key switch failed");
         if (p->njobs_lin) launch_linear(p->lin, p->njobs_lin, st);
         return check_launch("batched lvl02 gate launch");
     }
-    lvl1_launch(p->ks, p->njobs_ks, p->br, p->njobs_br, p->lin, p->njobs_lin, p->c, st);
+    if (!lvl1_launch(p->ks, p->njobs_ks, p->ks_ws, p->br, p->njobs_br, p->lin, p->njobs_lin, p->c, st))
+        return fail(CUFHE_ERR_HIP, "GEMM key switch failed");
     return check_launch("batched gate launch");
 }
 
@@ -608,6 +647,8 @@ int cufhe_initialize_lvl01_ex(const uint32_t* bk, size_t bk_words, int backend, 
                          (1u << (KS::basebit - 1)) * (KS::targetP::n + 1)
                    : TFHEpp::kKSKWords<KS>;
     if (rc == CUFHE_OK) rc = load_ksk(&s.k1.ksk, ksk, ksk_words, ksk_want);
+    if (rc == CUFHE_OK && ks_gemm_prepare(s.k1.ksg, s.k1.ksk, 1, subset_ksk != 0, nullptr) != 0)
+        rc = fail(CUFHE_ERR_HIP, "key-switching key byte planes");
     if (rc != CUFHE_OK) {
         free_keys(s.k1);
         return rc;
@@ -631,6 +672,8 @@ int cufhe_initialize_lvl02(const uint64_t* bk, size_t bk_words, int key_bundle,
     free_keys(s.k2);
     rc = load_bk<TFHEpp::lvl02param>(s.k2, s.t2, bk, bk_words, key_bundle != 0);
     if (rc == CUFHE_OK) rc = load_ksk(&s.k2.ksk, ksk, ksk_words, TFHEpp::kKSKWords<TFHEpp::lvl20param>);
+    if (rc == CUFHE_OK && ks_gemm_prepare(s.k2.ksg, s.k2.ksk, 2, false, nullptr) != 0)
+        rc = fail(CUFHE_ERR_HIP, "key-switching key byte planes");
     if (rc != CUFHE_OK) {
         free_keys(s.k2);
         return rc;
@@ -844,6 +887,11 @@ int cufhe_plan_create_lvl1(const cufhe_gate_desc* gates, int n, uint32_t* d_pool
     p->njobs_ks = static_cast<int>(jobs.ks.size());
     p->njobs_br = static_cast<int>(jobs.br.size());
     p->njobs_lin = static_cast<int>(jobs.lin.size());
+    rc = plan_alloc_ks_ws(p);
+    if (rc != CUFHE_OK) {
+        cufhe_plan_destroy(p);
+        return rc;
+    }
     *out = p;
     return CUFHE_OK;
 }
@@ -916,6 +964,11 @@ int cufhe_plan_create_lvl01(const cufhe_gate_desc* gates, int n, uint32_t* d_poo
     p->njobs_ks = static_cast<int>(ks.size());
     p->njobs_br = static_cast<int>(br.size());
     p->njobs_lin = static_cast<int>(lin.size());
+    rc = plan_alloc_ks_ws(p);
+    if (rc != CUFHE_OK) {
+        cufhe_plan_destroy(p);
+        return rc;
+    }
     *out = p;
     return CUFHE_OK;
 }
@@ -1076,6 +1129,11 @@ int cufhe_plan_create_lvl02(const cufhe_gate_desc* gates, int n, uint32_t* d_poo
     p->njobs_ks = static_cast<int>(ks.size());
     p->njobs_br = static_cast<int>(br.size());
     p->njobs_lin = static_cast<int>(lin.size());
+    rc = plan_alloc_ks_ws(p);
+    if (rc != CUFHE_OK) {
+        cufhe_plan_destroy(p);
+        return rc;
+    }
     *out = p;
     return CUFHE_OK;
 }
@@ -1117,6 +1175,7 @@ int cufhe_plan_destroy(cufhe_plan* p)
     (void)hipFree(p->br);
     (void)hipFree(p->lin);
     (void)hipFree(p->ws);
+    (void)hipFree(p->ks_ws);
     delete p;
     return CUFHE_OK;
 }
@@ -1144,7 +1203,7 @@ int cufhe_plan_run_timed(cufhe_plan* p, void* stream, float* kernel_ms)
                                                 static_cast<uint64_t>(lvl2param::μ), s.k2.fk, st);
         }
         CUFHE_HIP_TRY(hipEventRecord(e[1], st));
-        if (p->njobs_ks) launch_ks_batch(2, false, p->ks, p->njobs_ks, s.k2.ksk, st);
+        (void)ks_lvl2(p->ks, p->njobs_ks, p->ks_ws, st);
         CUFHE_HIP_TRY(hipEventRecord(e[2], st));
         if (p->njobs_lin) launch_linear(p->lin, p->njobs_lin, st);
         CUFHE_HIP_TRY(hipEventRecord(e[3], st));
@@ -1172,10 +1231,10 @@ int cufhe_plan_run_timed(cufhe_plan* p, void* stream, float* kernel_ms)
         // BR-IKS order; reported as [0] key switch, [1] rotation, [2] combine
         br_lvl1(p->br, p->njobs_br, p->c, static_cast<uint32_t>(lvl1param::μ), st);
         CUFHE_HIP_TRY(hipEventRecord(ev[1], st));
-        ks_lvl1(p->ks, p->njobs_ks, st);
+        (void)ks_lvl1(p->ks, p->njobs_ks, p->ks_ws, st);
     }
     else {
-        ks_lvl1(p->ks, p->njobs_ks, st);
+        (void)ks_lvl1(p->ks, p->njobs_ks, p->ks_ws, st);
         CUFHE_HIP_TRY(hipEventRecord(ev[1], st));
         br_lvl1(p->br, p->njobs_br, p->c, static_cast<uint32_t>(lvl1param::μ), st);
     }
@@ -1224,7 +1283,7 @@ int cufhe_identity_key_switch_lvl10(uint32_t* d_out, const uint32_t* d_in, int n
     KsJob* d = nullptr;
     CUFHE_HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&d), n * sizeof(KsJob), st));
     CUFHE_HIP_TRY(hipMemcpyAsync(d, jobs.data(), n * sizeof(KsJob), hipMemcpyHostToDevice, st));
-    ks_lvl1(d, n, st);
+    (void)ks_lvl1(d, n, nullptr, st);
     CUFHE_HIP_TRY(hipFreeAsync(d, st));
     CUFHE_HIP_TRY(hipStreamSynchronize(st));
     return check_launch("identity key switch");

@@ -110,6 +110,7 @@ struct cufhe_server {
     BrJob* d_br = nullptr;
     LinJob* d_lin = nullptr;
     uint32_t* d_ws = nullptr;
+    void* d_ksws = nullptr;  // GEMM key-switch workspace (max_boot jobs)
     KsJob* h_ks = nullptr;
     BrJob* h_br = nullptr;
     LinJob* h_lin = nullptr;
@@ -218,8 +219,9 @@ void worker_loop(cufhe_server* S)
         if (e == hipSuccess) {
             // c = 0: choose per round (small rounds use one job per workgroup)
             const int c = S->c ? S->c : lvl1_pick_c(0, nbr);
-            lvl1_launch(S->d_ks, nks, S->d_br, nbr, S->d_lin, nlin, c, S->st);
-            e = hipGetLastError();
+            if (!lvl1_launch(S->d_ks, nks, S->d_ksws, S->d_br, nbr, S->d_lin, nlin, c, S->st))
+                e = hipErrorLaunchFailure;
+            if (e == hipSuccess) e = hipGetLastError();
         }
         for (auto& r : finishing) {
             for (size_t o = 0; o < r->outputs.size() && e == hipSuccess; o++)
@@ -254,6 +256,7 @@ void free_server(cufhe_server* S)
     (void)hipFree(S->d_br);
     (void)hipFree(S->d_lin);
     (void)hipFree(S->d_ws);
+    (void)hipFree(S->d_ksws);
     (void)hipHostFree(S->h_ks);
     (void)hipHostFree(S->h_br);
     (void)hipHostFree(S->h_lin);
@@ -290,6 +293,8 @@ int cufhe_server_create(size_t pool_bytes, int max_batch_bootstraps, int jobs_pe
         e = hipMalloc(reinterpret_cast<void**>(&S->d_ws),
                       lvl1_workspace_words(max_batch_bootstraps, max_batch_bootstraps / 2) *
                           sizeof(uint32_t));
+    if (e == hipSuccess && lvl1_ks_workspace_bytes(max_batch_bootstraps))
+        e = hipMalloc(&S->d_ksws, lvl1_ks_workspace_bytes(max_batch_bootstraps));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&S->h_ks), mb * sizeof(KsJob), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&S->h_br), mb * sizeof(BrJob), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&S->h_lin), mb * sizeof(LinJob), hipHostMallocDefault);
