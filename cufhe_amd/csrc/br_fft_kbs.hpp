@@ -160,7 +160,7 @@ __device__ __forceinline__ void blind_rotate_kbs(KbsShared<C>& sm, int slot, int
                 for (int r = 0; r < R; r++)
                     z[r] = cplx{static_cast<double>(Gadget<P>::digit(acc[r], q)),
                                 static_cast<double>(Gadget<P>::digit(acc[R + r], q))};
-                fft_forward<H, CUFHE_KBS_SWZ != 0>(z, my_spec, tb);
+                fft_forward<H, CUFHE_KBS_SWZ != 0, true>(z, my_spec, tb);
 #pragma unroll
                 for (int d = 0; d < R; d++) lds_store(&my_spec[d * 64 + lane], z[d]);
             }
@@ -198,7 +198,7 @@ __device__ __forceinline__ void blind_rotate_kbs(KbsShared<C>& sm, int slot, int
             }
             __syncthreads();  // partner done reading this wave's spectrum
         }
-        fft_inverse<H, CUFHE_KBS_SWZ != 0>(accj, my_spec, tb);
+        fft_inverse<H, CUFHE_KBS_SWZ != 0, true>(accj, my_spec, tb);
         constexpr double scale = kTorusScale<uint32_t> / H;
 #pragma unroll
         for (int r = 0; r < R; r++) {

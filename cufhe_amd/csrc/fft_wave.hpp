@@ -185,6 +185,26 @@ __device__ __forceinline__ int fft_t2(int row, int c, int x)
     return row * 64 + ((SWZ ? (c ^ (row & 1)) : c) << 3) + x;
 }
 
+// Wave issue priority (s_setprio) for transforms run by many waves that
+// meet at workgroup barriers: a transform starts at priority 3 and steps down
+// after each transpose, so the waves that are behind get issue slots first
+// and the workgroup reaches its next barrier together (the arbiter otherwise
+// favours the oldest waves and the youngest set the pace).
+template <bool ON>
+__device__ __forceinline__ void wave_prio(int)
+{
+}
+template <>
+__device__ __forceinline__ void wave_prio<true>(int p)
+{
+    switch (p) {
+        case 3: __builtin_amdgcn_s_setprio(3); break;
+        case 2: __builtin_amdgcn_s_setprio(2); break;
+        case 1: __builtin_amdgcn_s_setprio(1); break;
+        default: __builtin_amdgcn_s_setprio(0); break;
+    }
+}
+
 // Tables of the folded transform (device memory, built at init), H = 64 R:
 //   twl[k2*64 + l] = exp(i*pi*l/N) * exp(-2*pi*i*l*k2/H)   (lane twist merged
 //                    with the step-2 twiddle), k2 < R, l < 64
@@ -202,12 +222,13 @@ struct FFTTables {
 // (untwisted). Out: spectrum Z[k] = sum_m (p[m] + i p[m+H]) zeta_k^m with
 // zeta_k = exp(i*pi*(1-4k)/N), in the register layout described above.
 // lds: H complex scratch owned by this wave.
-template <int H, bool SWZ = false>
+template <int H, bool SWZ = false, bool PRIO = false>
 __device__ __forceinline__ void fft_forward(cplx (&x)[H / 64], cplx* lds, const FFTTables& tb)
 {
     constexpr int R = H / 64;
     constexpr int G = R / 8;
     const int lane = threadIdx.x & 63;
+    wave_prio<PRIO>(3);
 #pragma unroll
     for (int r = 1; r < R; r++) x[r] = cmul(x[r], tb.cr[r]);
     dft_reg<R, false>(x);
@@ -222,6 +243,7 @@ __device__ __forceinline__ void fft_forward(cplx (&x)[H / 64], cplx* lds, const 
 #pragma unroll
         for (int b = 0; b < 8; b++) x[g * 8 + b] = lds_load(&lds[fft_t1<SWZ>(g * 8 + q, 8 * b + lo)]);
     wave_lds_sync();
+    wave_prio<PRIO>(2);
 #pragma unroll
     for (int g = 0; g < G; g++) {
         dft_reg<8, false>(&x[g * 8]);
@@ -236,19 +258,22 @@ __device__ __forceinline__ void fft_forward(cplx (&x)[H / 64], cplx* lds, const 
 #pragma unroll
         for (int a = 0; a < 8; a++) x[g * 8 + a] = lds_load(&lds[fft_t2<SWZ>(g * 8 + q, lo, a ^ lo)]);
     wave_lds_sync();
+    wave_prio<PRIO>(1);
 #pragma unroll
     for (int g = 0; g < G; g++) dft_reg<8, false>(&x[g * 8]);
+    wave_prio<PRIO>(0);
 }
 
 // Inverse folded transform without the 1/H factor: spectrum in the layout
 // fft_forward produces; result x[r] = H * (p[m] + i p[m+H]), m = lane + 64 r.
-template <int H, bool SWZ = false>
+template <int H, bool SWZ = false, bool PRIO = false>
 __device__ __forceinline__ void fft_inverse(cplx (&x)[H / 64], cplx* lds, const FFTTables& tb)
 {
     constexpr int R = H / 64;
     constexpr int G = R / 8;
     const int lane = threadIdx.x & 63;
     const int lo = lane & 7, q = lane >> 3;
+    wave_prio<PRIO>(3);
 #pragma unroll
     for (int g = 0; g < G; g++) {
         dft_reg<8, true>(&x[g * 8]);
@@ -261,6 +286,7 @@ __device__ __forceinline__ void fft_inverse(cplx (&x)[H / 64], cplx* lds, const 
 #pragma unroll
         for (int c = 0; c < 8; c++) x[g * 8 + c] = lds_load(&lds[fft_t2<SWZ>(g * 8 + q, c, lo ^ c)]);
     wave_lds_sync();
+    wave_prio<PRIO>(2);
 #pragma unroll
     for (int g = 0; g < G; g++) {
         // undo W_64^(a*c): lane low bits are a here, register index c
@@ -274,11 +300,13 @@ __device__ __forceinline__ void fft_inverse(cplx (&x)[H / 64], cplx* lds, const 
 #pragma unroll
     for (int k2 = 0; k2 < R; k2++) x[k2] = lds_load(&lds[fft_t1<SWZ>(k2, lane)]);
     wave_lds_sync();
+    wave_prio<PRIO>(1);
 #pragma unroll
     for (int k2 = 0; k2 < R; k2++) x[k2] = cmulc(x[k2], tb.twl[k2 * 64 + lane]);
     dft_reg<R, true>(x);
 #pragma unroll
     for (int r = 1; r < R; r++) x[r] = cmulc(x[r], tb.cr[r]);
+    wave_prio<PRIO>(0);
 }
 
 }  // namespace cufhe_amd
