@@ -218,6 +218,163 @@ struct FFTTables {
     const cplx* cr;
 };
 
+// Stages of the transforms for the paired forms (fft_forward2 /
+// fft_inverse2): the operations of fft_forward / fft_inverse split at the
+// transposes, so a polynomial's spectrum is bit-identical whichever form
+// transformed it (no expression here can be contracted differently).
+namespace fft_stage {
+
+// Scratch slot layouts of the transposes (16-byte elements). LAY 0 and 1 are
+// fft_t1/fft_t2 without / with the XOR swizzle; LAY 2 pads rows to 72
+// elements and step-2 sub-rows to 9, which is conflict-free for both access
+// patterns of both transposes (a 16-lane group's two rows land 8 elements =
+// half a bank sweep apart) and needs no per-register lane XOR, so one
+// address register serves all eight accesses of a pattern. Scratch for LAY 2
+// is kScratchPad(H) elements.
+constexpr int kPadRow = 72;
+__host__ __device__ constexpr int kScratchPad(int H) { return H / 64 * kPadRow; }
+// step 1: element (row, col)
+template <int LAY>
+__device__ __forceinline__ int slot_t1(int row, int col)
+{
+    if constexpr (LAY == 2)
+        return row * kPadRow + col;
+    else
+        return fft_t1<LAY == 1>(row, col);
+}
+// step 2: element (row, c, x) of the 8 x 8 transpose of lane group `row`:
+// lane x of the group holds register c before, lane c holds register x after
+template <int LAY>
+__device__ __forceinline__ int slot_t2(int row, int c, int x)
+{
+    if constexpr (LAY == 2)
+        return row * kPadRow + c * 9 + x;
+    else
+        return fft_t2<LAY == 1>(row, c, x ^ c);
+}
+
+template <int R>
+__device__ __forceinline__ void fwd_a(cplx (&x)[R], const FFTTables& tb, int lane)
+{
+#pragma unroll
+    for (int r = 1; r < R; r++) x[r] = cmul(x[r], tb.cr[r]);
+    dft_reg<R, false>(x);
+#pragma unroll
+    for (int k2 = 0; k2 < R; k2++) x[k2] = cmul(x[k2], tb.twl[k2 * 64 + lane]);
+}
+template <int R, int LAY>
+__device__ __forceinline__ void t1_store(const cplx (&x)[R], cplx* lds, int lane)
+{
+#pragma unroll
+    for (int k2 = 0; k2 < R; k2++) lds_store(&lds[slot_t1<LAY>(k2, lane)], x[k2]);
+}
+template <int R, int LAY>
+__device__ __forceinline__ void t1_load(cplx (&x)[R], const cplx* lds, int lane)
+{
+    const int lo = lane & 7, q = lane >> 3;
+#pragma unroll
+    for (int g = 0; g < R / 8; g++)
+#pragma unroll
+        for (int b = 0; b < 8; b++) x[g * 8 + b] = lds_load(&lds[slot_t1<LAY>(g * 8 + q, 8 * b + lo)]);
+}
+template <int R>
+__device__ __forceinline__ void fwd_b(cplx (&x)[R], const FFTTables& tb, int lane)
+{
+    const int lo = lane & 7;
+#pragma unroll
+    for (int g = 0; g < R / 8; g++) {
+        dft_reg<8, false>(&x[g * 8]);
+#pragma unroll
+        for (int c = 1; c < 8; c++) x[g * 8 + c] = cmul(x[g * 8 + c], tb.tw64[c * 8 + lo]);
+    }
+}
+template <int R, int LAY>
+__device__ __forceinline__ void t2_store_cx(const cplx (&x)[R], cplx* lds, int lane)
+{
+    const int lo = lane & 7, q = lane >> 3;
+#pragma unroll
+    for (int g = 0; g < R / 8; g++)
+#pragma unroll
+        for (int c = 0; c < 8; c++) lds_store(&lds[slot_t2<LAY>(g * 8 + q, c, lo)], x[g * 8 + c]);
+}
+template <int R, int LAY>
+__device__ __forceinline__ void t2_load_xc(cplx (&x)[R], const cplx* lds, int lane)
+{
+    const int lo = lane & 7, q = lane >> 3;
+#pragma unroll
+    for (int g = 0; g < R / 8; g++)
+#pragma unroll
+        for (int a = 0; a < 8; a++) x[g * 8 + a] = lds_load(&lds[slot_t2<LAY>(g * 8 + q, lo, a)]);
+}
+template <int R>
+__device__ __forceinline__ void fwd_c(cplx (&x)[R])
+{
+#pragma unroll
+    for (int g = 0; g < R / 8; g++) dft_reg<8, false>(&x[g * 8]);
+}
+template <int R>
+__device__ __forceinline__ void inv_a(cplx (&x)[R])
+{
+#pragma unroll
+    for (int g = 0; g < R / 8; g++) dft_reg<8, true>(&x[g * 8]);
+}
+template <int R, int LAY>
+__device__ __forceinline__ void t2_store_xc(const cplx (&x)[R], cplx* lds, int lane)
+{
+    const int lo = lane & 7, q = lane >> 3;
+#pragma unroll
+    for (int g = 0; g < R / 8; g++)
+#pragma unroll
+        for (int a = 0; a < 8; a++) lds_store(&lds[slot_t2<LAY>(g * 8 + q, lo, a)], x[g * 8 + a]);
+}
+template <int R, int LAY>
+__device__ __forceinline__ void t2_load_cx(cplx (&x)[R], const cplx* lds, int lane)
+{
+    const int lo = lane & 7, q = lane >> 3;
+#pragma unroll
+    for (int g = 0; g < R / 8; g++)
+#pragma unroll
+        for (int c = 0; c < 8; c++) x[g * 8 + c] = lds_load(&lds[slot_t2<LAY>(g * 8 + q, c, lo)]);
+}
+template <int R>
+__device__ __forceinline__ void inv_b(cplx (&x)[R], const FFTTables& tb, int lane)
+{
+    const int lo = lane & 7;
+#pragma unroll
+    for (int g = 0; g < R / 8; g++) {
+        // undo W_64^(a*c): lane low bits are a here, register index c
+#pragma unroll
+        for (int c = 1; c < 8; c++) x[g * 8 + c] = cmulc(x[g * 8 + c], tb.tw64[c * 8 + lo]);
+        dft_reg<8, true>(&x[g * 8]);
+    }
+}
+template <int R, int LAY>
+__device__ __forceinline__ void t1_store_inv(const cplx (&x)[R], cplx* lds, int lane)
+{
+    const int lo = lane & 7, q = lane >> 3;
+#pragma unroll
+    for (int g = 0; g < R / 8; g++)
+#pragma unroll
+        for (int b = 0; b < 8; b++) lds_store(&lds[slot_t1<LAY>(g * 8 + q, 8 * b + lo)], x[g * 8 + b]);
+}
+template <int R, int LAY>
+__device__ __forceinline__ void t1_load_inv(cplx (&x)[R], const cplx* lds, int lane)
+{
+#pragma unroll
+    for (int k2 = 0; k2 < R; k2++) x[k2] = lds_load(&lds[slot_t1<LAY>(k2, lane)]);
+}
+template <int R>
+__device__ __forceinline__ void inv_c(cplx (&x)[R], const FFTTables& tb, int lane)
+{
+#pragma unroll
+    for (int k2 = 0; k2 < R; k2++) x[k2] = cmulc(x[k2], tb.twl[k2 * 64 + lane]);
+    dft_reg<R, true>(x);
+#pragma unroll
+    for (int r = 1; r < R; r++) x[r] = cmulc(x[r], tb.cr[r]);
+}
+
+}  // namespace fft_stage
+
 // Forward folded transform. In: x[r] = p[m] + i p[m+H] for m = lane + 64 r
 // (untwisted). Out: spectrum Z[k] = sum_m (p[m] + i p[m+H]) zeta_k^m with
 // zeta_k = exp(i*pi*(1-4k)/N), in the register layout described above.
@@ -307,6 +464,72 @@ __device__ __forceinline__ void fft_inverse(cplx (&x)[H / 64], cplx* lds, const 
 #pragma unroll
     for (int r = 1; r < R; r++) x[r] = cmulc(x[r], tb.cr[r]);
     wave_prio<PRIO>(0);
+}
+
+// Two forward transforms by one wave through ONE scratch buffer: the
+// transposes take turns on the scratch while the other transform's register
+// stage runs, so the two dependency chains overlap.
+template <int H, int LAY = 2>
+__device__ __forceinline__ void fft_forward2(cplx (&x)[H / 64], cplx (&y)[H / 64], cplx* lds,
+                                             const FFTTables& tb)
+{
+    using namespace fft_stage;
+    constexpr int R = H / 64;
+    const int lane = threadIdx.x & 63;
+    fwd_a<R>(x, tb, lane);
+    t1_store<R, LAY>(x, lds, lane);
+    fwd_a<R>(y, tb, lane);
+    wave_lds_sync();
+    t1_load<R, LAY>(x, lds, lane);
+    wave_lds_sync();
+    t1_store<R, LAY>(y, lds, lane);
+    fwd_b<R>(x, tb, lane);
+    wave_lds_sync();
+    t1_load<R, LAY>(y, lds, lane);
+    wave_lds_sync();
+    t2_store_cx<R, LAY>(x, lds, lane);
+    fwd_b<R>(y, tb, lane);
+    wave_lds_sync();
+    t2_load_xc<R, LAY>(x, lds, lane);
+    wave_lds_sync();
+    t2_store_cx<R, LAY>(y, lds, lane);
+    fwd_c<R>(x);
+    wave_lds_sync();
+    t2_load_xc<R, LAY>(y, lds, lane);
+    wave_lds_sync();
+    fwd_c<R>(y);
+}
+
+// Two inverse transforms by one wave through one scratch buffer (as above).
+template <int H, int LAY = 2>
+__device__ __forceinline__ void fft_inverse2(cplx (&x)[H / 64], cplx (&y)[H / 64], cplx* lds,
+                                             const FFTTables& tb)
+{
+    using namespace fft_stage;
+    constexpr int R = H / 64;
+    const int lane = threadIdx.x & 63;
+    inv_a<R>(x);
+    t2_store_xc<R, LAY>(x, lds, lane);
+    inv_a<R>(y);
+    wave_lds_sync();
+    t2_load_cx<R, LAY>(x, lds, lane);
+    wave_lds_sync();
+    t2_store_xc<R, LAY>(y, lds, lane);
+    inv_b<R>(x, tb, lane);
+    wave_lds_sync();
+    t2_load_cx<R, LAY>(y, lds, lane);
+    wave_lds_sync();
+    t1_store_inv<R, LAY>(x, lds, lane);
+    inv_b<R>(y, tb, lane);
+    wave_lds_sync();
+    t1_load_inv<R, LAY>(x, lds, lane);
+    wave_lds_sync();
+    t1_store_inv<R, LAY>(y, lds, lane);
+    inv_c<R>(x, tb, lane);
+    wave_lds_sync();
+    t1_load_inv<R, LAY>(y, lds, lane);
+    wave_lds_sync();
+    inv_c<R>(y, tb, lane);
 }
 
 }  // namespace cufhe_amd

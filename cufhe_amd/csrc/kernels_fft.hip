@@ -2,6 +2,7 @@
 // per-gate kernels, key switching, MUX combine, and FFNT entry points.
 #include "br_fft.hpp"
 #include "br_fft_kbs.hpp"
+#include "br_fft_kbw.hpp"
 #include "keyswitch.hpp"
 #include "launch.hpp"
 
@@ -171,6 +172,57 @@ __global__ __launch_bounds__(128 * C, 1) void k_br_fft_kbs(const BrJob* __restri
             trlwe_store<P>(acc, j, static_cast<uint32_t*>(jb.out));
         else
             sample_extract_store<P>(acc, j, static_cast<uint32_t*>(jb.out));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched key-bundle rotation at lvl1, one wave per job, key shared by the
+// workgroup's C jobs through an LDS ring (br_fft_kbw.hpp). One workgroup of C
+// waves per CU (C = 8: 2 waves/SIMD, 130 KiB LDS).
+template <int C>
+__global__ __launch_bounds__(64 * C, 1) void k_br_fft_kbw(const BrJob* __restrict__ jobs,
+                                                           int njobs, uint32_t mu,
+                                                           FourierKeys<lvl1param> K)
+{
+    using P = lvl1param;
+    constexpr int N = P::n, H = N / 2, R = H / 64, n0 = lvl0param::n;
+    constexpr int NT = 64 * C;
+    __shared__ KbwShared<C> sm;
+    const int slot = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int t = threadIdx.x;
+    for (int i = t; i < R * 64; i += NT) sm.twl[i] = K.fft.twl[i];
+    for (int i = t; i < 64; i += NT) {
+        sm.tw64[i] = K.fft.tw64[i];
+        sm.e2n_lo[i] = K.e2n[i];
+    }
+    for (int i = t; i < R; i += NT) sm.cr[i] = K.fft.cr[i];
+    for (int i = t; i < 2 * N / 64; i += NT) sm.e2n_hi[i] = K.e2n[64 * i];
+    const int raw = blockIdx.x * C + slot;
+    const bool live = raw < njobs;
+    const BrJob jb = jobs[live ? raw : njobs - 1];
+    constexpr uint32_t rnd = kMaskRound<P>;
+    for (int i = t & 63; i <= n0; i += 64) {
+        uint32_t v = static_cast<uint32_t>(jb.ca) * jb.in0[i];
+        if (jb.in1) v += static_cast<uint32_t>(jb.cb) * jb.in1[i];
+        if (i < n0)
+            sm.abar[slot][i] = static_cast<uint16_t>(mod_switch<P>(v + rnd));
+        else
+            sm.bar[slot] = 2 * N - mod_switch<P>(v + jb.offset);
+    }
+    __syncthreads();
+    uint32_t acc[2][16];
+    blind_rotate_kbw<C>(sm, slot, mu, acc, K.bk);
+    if (live) {
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            AccStore<P> a;
+#pragma unroll
+            for (int r = 0; r < 16; r++) a.reg[r] = acc[j][r];
+            if (jb.flags & kBrOutTrlwe)
+                trlwe_store<P>(a, j, static_cast<uint32_t*>(jb.out));
+            else
+                sample_extract_store<P>(a, j, static_cast<uint32_t*>(jb.out));
+        }
     }
 }
 
@@ -461,7 +513,11 @@ void launch_br_fft(const BrJob* jobs, int njobs, int c, typename P::T mu,
                                    K);
             break;
         case 8:
-            if constexpr (sizeof(typename P::T) == 4)
+            // key bundle: one wave per job (k_br_fft_kbw); otherwise two
+            if constexpr (KB && sizeof(typename P::T) == 4)
+                hipLaunchKernelGGL((k_br_fft_kbw<8>), grid, dim3(64 * 8), 0, st, jobs, njobs, mu,
+                                   K);
+            else if constexpr (sizeof(typename P::T) == 4)
                 hipLaunchKernelGGL((k_br_fft<P, KB, 8>), grid, block, 0, st, jobs, njobs, mu,
                                    K);
             break;

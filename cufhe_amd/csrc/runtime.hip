@@ -498,11 +498,12 @@ int lvl1_pick_c(int c, int nbr)
     constexpr int kSharedMinJobs = 6 * 256;
     if (c == 0 && nbr >= 0 && nbr < kSharedMinJobs) c = 1;
     // defaults from the MI355X sweeps (lvl1, 4096 gates, BR ms per launch):
-    //   FFT KB : C=1 24.0, 2 24.5, 4 34.2, 6 (shared key) 18.1, 8 38.9
+    //   FFT KB : C=1 24.0, 2 24.5, 4 34.2, 6 (shared key, two waves per job)
+    //            17.6, 8 (shared key, one wave per job) 16.2
     //   NTT BB : C=1 18.1, 2 17.3, 4 20.3, 6 (shared key) 21.6
     // the 16-byte FFT key stream is what sharing pays for; the 4-byte NTT
     // key is not worth the extra barriers
-    if (c == 0) c = k.backend == CUFHE_BACKEND_NTT ? 2 : (k.kb ? 6 : 1);
+    if (c == 0) c = k.backend == CUFHE_BACKEND_NTT ? 2 : (k.kb ? 8 : 1);
     // 6: the kernels that share key chunks among six jobs through LDS (NTT
     // backend, all modes; FFT backend, key bundle)
     if (c == 6) return (k.backend == CUFHE_BACKEND_NTT || k.kb) ? 6 : -1;

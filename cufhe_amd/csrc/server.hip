@@ -275,7 +275,7 @@ int cufhe_server_create(size_t pool_bytes, int max_batch_bootstraps, int jobs_pe
     *server = nullptr;
     if (!lvl1_loaded()) return set_error(CUFHE_ERR_STATE, "lvl1 keys not loaded (cufhe_initialize_lvl01)");
     const int c = jobs_per_group == 0 ? 0 : lvl1_pick_c(jobs_per_group, -1);
-    if (c < 0) return set_error(CUFHE_ERR_UNSUPPORTED, "jobs_per_group must be 0, 1, 2, 4 or 8");
+    if (c < 0) return set_error(CUFHE_ERR_UNSUPPORTED, "jobs_per_group must be 0, 1, 2, 4, 6 or 8");
     if (max_batch_bootstraps <= 0) max_batch_bootstraps = 8192;
     const size_t wire_bytes = static_cast<size_t>(kWireWords1) * sizeof(uint32_t);
     if (pool_bytes < wire_bytes) return set_error(CUFHE_ERR_ARG, "pool smaller than one ciphertext");

@@ -310,7 +310,8 @@ def test_batch_plan_matches_plain_form(engine, keys_lvl1):
     pool[: 3 * n] = pool_ct
     gates = [(ops[i], 3 * n + i, i, n + i, 2 * n + i) for i in range(n)]
     res = {}
-    for c, graph in [(1, False), (2, False), (4, False), (6, False), (2, True), (6, True)]:
+    for c, graph in [(1, False), (2, False), (4, False), (6, False), (8, False), (2, True),
+                     (6, True), (8, True)]:
         dpool = C.DeviceBuffer.from_numpy(pool)
         plan = C.Plan(gates, dpool, jobs_per_group=c, use_graph=graph)
         for _ in range(2 if graph else 1):
