@@ -144,7 +144,7 @@ __device__ __forceinline__ void blind_rotate_kbw(KbwShared<C>& sm, int slot, uin
                 z0[r] = cplx{digit((dg[0][r] >> sh) & 255u), digit((dg[0][r] >> (sh + 16)) & 255u)};
                 z1[r] = cplx{digit((dg[1][r] >> sh) & 255u), digit((dg[1][r] >> (sh + 16)) & 255u)};
             }
-            fft_forward2<H>(z0, z1, scratch, tables());
+            fft_forward2<H, 2, true>(z0, z1, scratch, tables());
             // component 1's spectrum waits in the (now idle) transform
             // scratch: 32 fewer VGPRs held through the multiply-accumulate
 #pragma unroll
@@ -193,7 +193,7 @@ __device__ __forceinline__ void blind_rotate_kbw(KbwShared<C>& sm, int slot, uin
                 load_chunk(g + 1 + PD, kv[(pair + 1) % PD]);
             }
         }
-        fft_inverse2<H>(accj[0], accj[1], scratch, tables());
+        fft_inverse2<H, 2, true>(accj[0], accj[1], scratch, tables());
         constexpr double scale = kTorusScale<uint32_t> / H;
 #pragma unroll
         for (int j = 0; j < 2; j++)

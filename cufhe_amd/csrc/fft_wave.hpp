@@ -469,13 +469,14 @@ __device__ __forceinline__ void fft_inverse(cplx (&x)[H / 64], cplx* lds, const 
 // Two forward transforms by one wave through ONE scratch buffer: the
 // transposes take turns on the scratch while the other transform's register
 // stage runs, so the two dependency chains overlap.
-template <int H, int LAY = 2>
+template <int H, int LAY = 2, bool PRIO = false>
 __device__ __forceinline__ void fft_forward2(cplx (&x)[H / 64], cplx (&y)[H / 64], cplx* lds,
                                              const FFTTables& tb)
 {
     using namespace fft_stage;
     constexpr int R = H / 64;
     const int lane = threadIdx.x & 63;
+    wave_prio<PRIO>(3);
     fwd_a<R>(x, tb, lane);
     t1_store<R, LAY>(x, lds, lane);
     fwd_a<R>(y, tb, lane);
@@ -487,27 +488,31 @@ __device__ __forceinline__ void fft_forward2(cplx (&x)[H / 64], cplx (&y)[H / 64
     wave_lds_sync();
     t1_load<R, LAY>(y, lds, lane);
     wave_lds_sync();
+    wave_prio<PRIO>(2);
     t2_store_cx<R, LAY>(x, lds, lane);
     fwd_b<R>(y, tb, lane);
     wave_lds_sync();
     t2_load_xc<R, LAY>(x, lds, lane);
     wave_lds_sync();
+    wave_prio<PRIO>(1);
     t2_store_cx<R, LAY>(y, lds, lane);
     fwd_c<R>(x);
     wave_lds_sync();
     t2_load_xc<R, LAY>(y, lds, lane);
     wave_lds_sync();
     fwd_c<R>(y);
+    wave_prio<PRIO>(0);
 }
 
 // Two inverse transforms by one wave through one scratch buffer (as above).
-template <int H, int LAY = 2>
+template <int H, int LAY = 2, bool PRIO = false>
 __device__ __forceinline__ void fft_inverse2(cplx (&x)[H / 64], cplx (&y)[H / 64], cplx* lds,
                                              const FFTTables& tb)
 {
     using namespace fft_stage;
     constexpr int R = H / 64;
     const int lane = threadIdx.x & 63;
+    wave_prio<PRIO>(3);
     inv_a<R>(x);
     t2_store_xc<R, LAY>(x, lds, lane);
     inv_a<R>(y);
@@ -519,17 +524,20 @@ __device__ __forceinline__ void fft_inverse2(cplx (&x)[H / 64], cplx (&y)[H / 64
     wave_lds_sync();
     t2_load_cx<R, LAY>(y, lds, lane);
     wave_lds_sync();
+    wave_prio<PRIO>(2);
     t1_store_inv<R, LAY>(x, lds, lane);
     inv_b<R>(y, tb, lane);
     wave_lds_sync();
     t1_load_inv<R, LAY>(x, lds, lane);
     wave_lds_sync();
+    wave_prio<PRIO>(1);
     t1_store_inv<R, LAY>(y, lds, lane);
     inv_c<R>(x, tb, lane);
     wave_lds_sync();
     t1_load_inv<R, LAY>(y, lds, lane);
     wave_lds_sync();
     inv_c<R>(y, tb, lane);
+    wave_prio<PRIO>(0);
 }
 
 }  // namespace cufhe_amd
