@@ -262,6 +262,24 @@ __device__ __forceinline__ void fwd_a(cplx (&x)[R], const FFTTables& tb, int lan
 #pragma unroll
     for (int k2 = 0; k2 < R; k2++) x[k2] = cmul(x[k2], tb.twl[k2 * 64 + lane]);
 }
+// fwd_a of two transforms with each twiddle read once
+template <int R>
+__device__ __forceinline__ void fwd_a2(cplx (&x)[R], cplx (&y)[R], const FFTTables& tb, int lane)
+{
+#pragma unroll
+    for (int r = 1; r < R; r++) {
+        x[r] = cmul(x[r], tb.cr[r]);
+        y[r] = cmul(y[r], tb.cr[r]);
+    }
+    dft_reg<R, false>(x);
+    dft_reg<R, false>(y);
+#pragma unroll
+    for (int k2 = 0; k2 < R; k2++) {
+        const cplx t = tb.twl[k2 * 64 + lane];
+        x[k2] = cmul(x[k2], t);
+        y[k2] = cmul(y[k2], t);
+    }
+}
 template <int R, int LAY>
 __device__ __forceinline__ void t1_store(const cplx (&x)[R], cplx* lds, int lane)
 {
@@ -372,6 +390,24 @@ __device__ __forceinline__ void inv_c(cplx (&x)[R], const FFTTables& tb, int lan
 #pragma unroll
     for (int r = 1; r < R; r++) x[r] = cmulc(x[r], tb.cr[r]);
 }
+// inv_c of two transforms with each twiddle read once
+template <int R>
+__device__ __forceinline__ void inv_c2(cplx (&x)[R], cplx (&y)[R], const FFTTables& tb, int lane)
+{
+#pragma unroll
+    for (int k2 = 0; k2 < R; k2++) {
+        const cplx t = tb.twl[k2 * 64 + lane];
+        x[k2] = cmulc(x[k2], t);
+        y[k2] = cmulc(y[k2], t);
+    }
+    dft_reg<R, true>(x);
+    dft_reg<R, true>(y);
+#pragma unroll
+    for (int r = 1; r < R; r++) {
+        x[r] = cmulc(x[r], tb.cr[r]);
+        y[r] = cmulc(y[r], tb.cr[r]);
+    }
+}
 
 }  // namespace fft_stage
 
@@ -477,9 +513,8 @@ __device__ __forceinline__ void fft_forward2(cplx (&x)[H / 64], cplx (&y)[H / 64
     constexpr int R = H / 64;
     const int lane = threadIdx.x & 63;
     wave_prio<PRIO>(3);
-    fwd_a<R>(x, tb, lane);
+    fwd_a2<R>(x, y, tb, lane);
     t1_store<R, LAY>(x, lds, lane);
-    fwd_a<R>(y, tb, lane);
     wave_lds_sync();
     t1_load<R, LAY>(x, lds, lane);
     wave_lds_sync();
@@ -532,11 +567,10 @@ __device__ __forceinline__ void fft_inverse2(cplx (&x)[H / 64], cplx (&y)[H / 64
     wave_lds_sync();
     wave_prio<PRIO>(1);
     t1_store_inv<R, LAY>(y, lds, lane);
-    inv_c<R>(x, tb, lane);
     wave_lds_sync();
     t1_load_inv<R, LAY>(y, lds, lane);
     wave_lds_sync();
-    inv_c<R>(y, tb, lane);
+    inv_c2<R>(x, y, tb, lane);
     wave_prio<PRIO>(0);
 }
 
