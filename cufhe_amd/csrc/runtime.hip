@@ -490,22 +490,55 @@ void lvl1_build(const cufhe_gate_desc* gates, int n, uint32_t* d_pool, uint32_t*
 
 bool lvl1_loaded() { return state().k1.loaded; }
 
+// Compute units of the current device (queried once per device).
+static int device_cu_count()
+{
+    static int cached[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int v = __atomic_load_n(&cached[dev], __ATOMIC_RELAXED);
+    if (v <= 0) {
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            v <= 0)
+            v = 256;
+        __atomic_store_n(&cached[dev], v, __ATOMIC_RELAXED);
+    }
+    return v;
+}
+
 int lvl1_pick_c(int c, int nbr)
 {
     const LevelKeys<lvl1param>& k = state().k1;
-    // a shared-key workgroup holds a whole CU: below ~6 jobs per CU the
-    // one-job workgroups spread over more CUs and finish sooner
-    constexpr int kSharedMinJobs = 6 * 256;
-    if (c == 0 && nbr >= 0 && nbr < kSharedMinJobs) c = 1;
-    // defaults from the MI355X sweeps (lvl1, 4096 gates, BR ms per launch):
-    //   FFT KB : C=1 24.0, 2 24.5, 4 34.2, 6 (shared key, two waves per job)
-    //            17.6, 8 (shared key, one wave per job) 16.2
-    //   NTT BB : C=1 18.1, 2 17.3, 4 20.3, 6 (shared key) 21.6
-    // the 16-byte FFT key stream is what sharing pays for; the 4-byte NTT
-    // key is not worth the extra barriers
-    if (c == 0) c = k.backend == CUFHE_BACKEND_NTT ? 2 : (k.kb ? 8 : 1);
+    if (c == 0) {
+        if (k.backend == CUFHE_BACKEND_NTT) {
+            // NTT block-binary sweep (4096 gates, BR ms): C=1 18.1, 2 17.3,
+            // 4 20.3, 6 (shared key) 21.6 -- the 4-byte NTT key is not worth
+            // the ring's barriers
+            c = 2;
+        }
+        else if (k.kb) {
+            // The shared-key kernels run one workgroup per CU and take a
+            // near-constant time per round of workgroups (MI355X, per round:
+            // 6 jobs/CU in 5.9 ms, 8 jobs/CU in 6.3 ms; the one-job kernel
+            // needs 7.5 ms even for a single round), so pick the one with
+            // the shorter rounds x time: 6 up to 1536 jobs and at 3072, 8 at
+            // 2048, 4096 and beyond. nbr < 0 (size unknown): 8.
+            if (nbr < 0) {
+                c = 8;
+            }
+            else {
+                const long cus = device_cu_count();
+                const long r6 = (nbr + 6 * cus - 1) / (6 * cus), r8 = (nbr + 8 * cus - 1) / (8 * cus);
+                c = r6 * 59 < r8 * 63 ? 6 : 8;
+            }
+        }
+        else {
+            c = 1;
+        }
+    }
     // 6: the kernels that share key chunks among six jobs through LDS (NTT
-    // backend, all modes; FFT backend, key bundle)
+    // backend, all modes; FFT backend, key bundle); 8 with the key bundle:
+    // one wave per job, key shared by eight
     if (c == 6) return (k.backend == CUFHE_BACKEND_NTT || k.kb) ? 6 : -1;
     if (c != 1 && c != 2 && c != 4 && c != 8) return -1;
     return c;
