@@ -36,10 +36,15 @@ namespace cufhe_amd {
 // LDS of one workgroup of C jobs. The arrays every wave addresses at fixed
 // offsets (key ring, tables) come first, below 64 KiB, so one lane-address
 // register plus the 16-bit instruction offset reaches all of them.
+// Registers of the key chunk: 2 for 8 jobs per workgroup (24 slices, 3 per
+// wave), 1 for 4 (12 slices, 3 per wave; two workgroups share a CU).
+template <int C>
+constexpr int kKbwRpc = C >= 8 ? 2 : 1;
+
 template <int C>
 struct KbwShared {
     static constexpr int N = lvl1param::n, H = N / 2, R = H / 64, n0 = lvl0param::n;
-    cplx ring[2][2][12][64];  // key chunks: [buffer][register of the pair][slice][bin]
+    cplx ring[2][kKbwRpc<C>][12][64];  // key chunks: [buffer][register][slice][bin]
     cplx twl[R * 64];         // transform tables (copies of FFTTables)
     cplx tw64[64];
     cplx cr[R];
@@ -59,11 +64,12 @@ __device__ __forceinline__ void blind_rotate_kbw(KbwShared<C>& sm, int slot, uin
     using P = lvl1param;
     constexpr int N = P::n, H = N / 2, R = H / 64, L = P::l, n0 = lvl0param::n;
     constexpr int W = C;                     // waves per workgroup
-    constexpr int NS = 24;                   // slices per chunk
+    constexpr int RPC = kKbwRpc<C>;          // registers per chunk
+    constexpr int NS = 12 * RPC;             // slices per chunk
     constexpr int SPW = (NS + W - 1) / W;    // slices loaded per wave and chunk
     constexpr int STEPS = n0 / 2;
     constexpr int PD = CUFHE_KBW_PD;
-    constexpr int CPD = R / 2;               // chunks per digit pass
+    constexpr int CPD = R / RPC;             // chunks per digit pass
     constexpr int CHUNKS = STEPS * L * CPD;  // chunk g = (it * L + q) * CPD + pair
     static_assert(R == 8 && P::Bgbit == 8 && L == 2, "lvl1 shapes");
     const int lane = threadIdx.x & 63;
@@ -87,7 +93,7 @@ __device__ __forceinline__ void blind_rotate_kbw(KbwShared<C>& sm, int slot, uin
     }
 
     // Key pipeline: slices of chunk g are held in kv[g % PD] until written.
-    // Slice t of a chunk: register d = 2 * pair + t / 12, ring slice t % 12.
+    // Slice t of a chunk: register d = RPC * pair + t / 12, ring slice t % 12.
     cplx kv[PD][SPW];
     auto load_chunk = [&](int g, cplx (&dst)[SPW]) {
         const int pair = g % CPD, q = (g / CPD) % L, it = g / (CPD * L);
@@ -95,7 +101,7 @@ __device__ __forceinline__ void blind_rotate_kbw(KbwShared<C>& sm, int slot, uin
         for (int k = 0; k < SPW; k++) {
             const int t = slot + k * W;
             if ((SPW * W == NS || t < NS) && g < CHUNKS)
-                dst[k] = *kbs_slice_ptr(bk, it, q, 2 * pair + t / 12, t % 12, lane);
+                dst[k] = *kbs_slice_ptr(bk, it, q, RPC * pair + t / 12, t % 12, lane);
         }
     };
     auto store_chunk = [&](int g, const cplx (&src)[SPW]) {
@@ -161,8 +167,8 @@ __device__ __forceinline__ void blind_rotate_kbw(KbwShared<C>& sm, int slot, uin
                 __syncthreads();  // chunk g in the ring; chunk g-1 consumed
                 const cplx (*ring)[12][64] = sm.ring[g & 1];
 #pragma unroll
-                for (int dd = 0; dd < 2; dd++) {
-                    const int d = 2 * pair + dd;
+                for (int dd = 0; dd < RPC; dd++) {
+                    const int d = RPC * pair + dd;
                     // all LDS operands of this register first (13 loads in
                     // flight), then the arithmetic
                     cplx kr[12];

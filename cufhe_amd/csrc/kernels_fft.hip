@@ -180,7 +180,7 @@ __global__ __launch_bounds__(128 * C, 1) void k_br_fft_kbs(const BrJob* __restri
 // workgroup's C jobs through an LDS ring (br_fft_kbw.hpp). One workgroup of C
 // waves per CU (C = 8: 2 waves/SIMD, 130 KiB LDS).
 template <int C>
-__global__ __launch_bounds__(64 * C, 1) void k_br_fft_kbw(const BrJob* __restrict__ jobs,
+__global__ __launch_bounds__(64 * C, 8 / C) void k_br_fft_kbw(const BrJob* __restrict__ jobs,
                                                            int njobs, uint32_t mu,
                                                            FourierKeys<lvl1param> K)
 {
