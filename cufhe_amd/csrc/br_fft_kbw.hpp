@@ -166,33 +166,68 @@ __device__ __forceinline__ void blind_rotate_kbw(KbwShared<C>& sm, int slot, uin
                 const int g = g0 + pair;
                 __syncthreads();  // chunk g in the ring; chunk g-1 consumed
                 const cplx (*ring)[12][64] = sm.ring[g & 1];
+                // W_8^(a d) of this chunk's registers, fetched (scalar loads)
+                // before any LDS load of the chunk: scalar loads complete out
+                // of order, so a wait for them would also drain every LDS
+                // load in flight
+                cplx w8v[RPC][3];
 #pragma unroll
                 for (int dd = 0; dd < RPC; dd++) {
+                    const uint32_t d = RPC * pair + dd;
+                    const uint32_t m[3] = {(x0g.a * d) & 7u, (x1g.a * d) & 7u, (x01g.a * d) & 7u};
+#pragma unroll
+                    for (int e = 0; e < 3; e++) w8v[dd][e] = cplx{kW8[m[e]][0], kW8[m[e]][1]};
+                }
+#pragma unroll
+                for (int dd = 0; dd < RPC; dd++)
+#pragma unroll
+                    for (int e = 0; e < 3; e++) asm volatile("" : "+s"(w8v[dd][e].x), "+s"(w8v[dd][e].y));
+                // (register, column) blocks of 6 key slices, software
+                // pipelined: the next block's LDS loads are issued before the
+                // current block's arithmetic, so LDS and ALUs overlap
+                constexpr int NB = RPC * 2;
+                cplx kb[2][6];
+                cplx D1[RPC];
+                auto load_block = [&](int blk, cplx (&dst)[6]) {
+                    const int dd = blk >> 1, col = blk & 1;
+                    if (col == 0) D1[dd] = lds_load(&scratch[(RPC * pair + dd) * 64 + lane]);
+#pragma unroll
+                    for (int t = 0; t < 6; t++) dst[t] = lds_load(&ring[dd][col * 6 + t][lane]);
+                };
+                auto xai = [](const cplx& base, const cplx& w) {
+                    const cplx v = cmul(base, w);  // as XaiGenShared::at
+                    return cplx{v.x - 1.0, v.y};
+                };
+                load_block(0, kb[0]);
+                cplx x0, x1, x01;
+#pragma unroll
+                for (int blk = 0; blk < NB; blk++) {
+                    const int dd = blk >> 1, col = blk & 1;
                     const int d = RPC * pair + dd;
-                    // all LDS operands of this register first (13 loads in
-                    // flight), then the arithmetic
-                    cplx kr[12];
-#pragma unroll
-                    for (int t = 0; t < 12; t++) kr[t] = lds_load(&ring[dd][t][lane]);
-                    const cplx D1 = lds_load(&scratch[d * 64 + lane]);
-                    const cplx x0 = x0g.at(d), x1 = x1g.at(d), x01 = x01g.at(d);
-#pragma unroll
-                    for (int col = 0; col < 2; col++) {
-                        cplx sacc = accj[col][d];
-#pragma unroll
-                        for (int comp = 0; comp < 2; comp++) {
-                            const cplx D = comp == 0 ? z0[d] : D1;
-                            const int s0 = (col * 2 + comp) * 3;
-                            cplx k = {comp == col ? hq : 0.0, 0.0};
-                            cmac(k, kr[s0 + 2], x1);
-                            cmac(k, kr[s0 + 1], x0);
-                            cmac(k, kr[s0 + 0], x01);
-                            cmac(sacc, D, k);
-                        }
-                        // keep this chunk's arithmetic ahead of the next barrier
-                        asm volatile("" : "+v"(sacc.x), "+v"(sacc.y));
-                        accj[col][d] = sacc;
+                    if (blk + 1 < NB) load_block(blk + 1, kb[(blk + 1) & 1]);
+                    if (col == 0) {
+                        x0 = xai(x0g.base, w8v[dd][0]);
+                        x1 = xai(x1g.base, w8v[dd][1]);
+                        x01 = xai(x01g.base, w8v[dd][2]);
                     }
+                    const cplx (&kr)[6] = kb[blk & 1];
+                    cplx sacc = accj[col][d];
+#pragma unroll
+                    for (int comp = 0; comp < 2; comp++) {
+                        const cplx D = comp == 0 ? z0[d] : D1[dd];
+                        const int s0 = comp * 3;
+                        cplx k = {comp == col ? hq : 0.0, 0.0};
+                        cmac(k, kr[s0 + 2], x1);
+                        cmac(k, kr[s0 + 1], x0);
+                        cmac(k, kr[s0 + 0], x01);
+                        cmac(sacc, D, k);
+                    }
+                    // keep this chunk's arithmetic ahead of the next barrier
+                    asm volatile("" : "+v"(sacc.x), "+v"(sacc.y));
+                    accj[col][d] = sacc;
+                    // the scheduler may not pull a later block's loads
+                    // ahead of this block's arithmetic
+                    __builtin_amdgcn_sched_barrier(0);
                 }
                 // refill: chunk g+1 into the other buffer, fetch chunk g+1+PD
                 store_chunk(g + 1, kv[(pair + 1) % PD]);
