@@ -10,9 +10,9 @@
 // Organisation (versus br_fft_kbs.hpp, two waves per job):
 //  * a wave holds both TRLWE components of its job's accumulator, transforms
 //    both digit polynomials of a gadget digit as one interleaved pair
-//    (fft_forward2: the two dependency chains overlap) and keeps both
-//    spectra in VGPRs, so no spectrum goes through LDS and no wave waits for
-//    a partner;
+//    (fft_forward2: the two dependency chains overlap) and keeps the
+//    spectra itself (component 0 in VGPRs, component 1 in its idle transform
+//    scratch), so no wave waits for a partner's spectrum;
 //  * the multiply-accumulate produces both output columns (16 complex
 //    accumulators) and the two inverse transforms again run as a pair;
 //  * C = 8 jobs per workgroup at 2 waves/SIMD (256 VGPRs per wave): a batch
@@ -22,7 +22,9 @@
 //    -- is fetched once per workgroup (wave w loads slices w, w+8, w+16 one
 //    chunk ahead into VGPRs), written into a two-chunk LDS ring and read by
 //    every wave: one workgroup barrier per chunk, 8 per step, no other
-//    barrier in the loop.
+//    barrier in the loop; within a chunk the (register, column) blocks are
+//    software pipelined (the next block's six slices are loaded while the
+//    current block is multiplied).
 #pragma once
 
 #include "br_fft_kbs.hpp"
