@@ -351,3 +351,36 @@ def _run_gates_plain_pool(keys, ops, pool_ct, n):
     st.synchronize()
     st.destroy()
     return None, None, None, do.download(np.uint32, (n, W1))
+
+
+def test_shared_key_kernels_agree_at_scale(engine, keys_lvl1):
+    """2100 key-bundle gates (several rounds of workgroups, a partial last
+    workgroup for every kernel): the one-job, the two-waves-per-job and the
+    one-wave-per-job rotations give identical ciphertexts, the default
+    kernel choice follows the round model, and every gate decrypts right."""
+    load_lvl1(keys_lvl1, True)
+    rng = np.random.default_rng(77)
+    n = 2100
+    names = ["NAND", "AND", "OR", "XOR", "XNOR", "NOR", "ANDNY", "ORYN"]
+    ops = [names[i] for i in rng.integers(0, len(names), n)]
+    bits = rng.integers(0, 2, 2 * n).astype(np.uint8)
+    pool = np.zeros((3 * n, W1), np.uint32)
+    pool[: 2 * n] = C.encrypt_bits(1, 901, keys_lvl1["k1"], bits)
+    gates = [(ops[i], 2 * n + i, i, n + i, 0) for i in range(n)]
+    res = {}
+    for c in (1, 6, 8, 0):
+        dpool = C.DeviceBuffer.from_numpy(pool)
+        plan = C.Plan(gates, dpool, jobs_per_group=c)
+        if c == 0:
+            # 2100 jobs: two rounds of 6-job workgroups (2 x 5.9 ms) against
+            # two rounds of 8-job ones (2 x 6.3 ms) on 256 CUs
+            assert plan.jobs_per_group in (6, 8)
+        plan.run()
+        C.check(C.lib().cufhe_synchronize(), "sync")
+        res[c] = dpool.download(np.uint32, pool.shape)[2 * n:]
+        plan.destroy()
+    for c in (6, 8, 0):
+        assert np.array_equal(res[c], res[1]), c
+    dec = C.decrypt_bits(1, keys_lvl1["k1"], res[1])
+    want = np.array([C.TRUTH[o](int(bits[i]), int(bits[n + i])) for i, o in enumerate(ops)], np.uint8)
+    assert np.array_equal(dec, want)
