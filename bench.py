@@ -124,6 +124,36 @@ def autotune(C, gates, dpool, cands):
     return min(tune, key=tune.get), tune
 
 
+def verify_chained(C, plan, dpool, ops, bits, B, k1, total):
+    """Decryption check over `total` gates outside the timed region: the
+    batch is run as consecutive layers, each layer's outputs becoming the
+    next layer's first inputs (second inputs fixed), every output decrypted
+    and compared with the plaintext evaluation. Returns (gates, failures,
+    layers)."""
+    import numpy as np
+
+    if total <= 0:
+        return 0, 0, 0
+    W = C.TLWE1_WORDS
+    tt = np.array([[C.TRUTH[o](x, y) for x in (0, 1) for y in (0, 1)] for o in ops], np.uint8)
+    xs = bits[:B].astype(np.uint8).copy()
+    ys = bits[B:2 * B].astype(np.uint8)
+    out = np.empty((B, W), np.uint32)
+    done = fails = layers = 0
+    while done < total:
+        plan.run()
+        C.check(C.lib().cufhe_synchronize(), "sync")
+        C.check(C.lib().cufhe_memcpy_d2h(C._ptr(out), dpool.at(2 * B * W * 4), out.nbytes, None), "d2h")
+        C.check(C.lib().cufhe_stream_synchronize(None), "sync")
+        xs = tt[np.arange(B), 2 * xs + ys]
+        fails += int(np.sum(C.decrypt_bits(1, k1, out) != xs))
+        dpool.upload(out)  # outputs -> wires [0, B): the next layer's first inputs
+        done += B
+        layers += 1
+    log("chained verification: %d gates over %d layers, %d decryption failures" % (done, layers, fails))
+    return done, fails, layers
+
+
 def measured_traffic(workload):
     """HBM bytes per launch of the workload's blind-rotation kernel, from the
     rocprofv3 FETCH_SIZE/WRITE_SIZE runs summarised under profiles/."""
@@ -186,6 +216,7 @@ def workload_kb4096(args):
     log("batched: %.1f gates/s, %.2f ms/step (ks %.2f ms, br %.2f ms)" %
         (value, ms_per_step, statistics.mean(ks_ms), statistics.mean(br_ms)))
     gms = graph_ms(C, gates, dpool, best_c)
+    verified, vfail, vlayers = verify_chained(C, plan, dpool, ops, bits, B, k1, args.verify_gates)
 
     P = min(args.plain_sample, B)
     W = C.TLWE1_WORDS * 4
@@ -238,6 +269,7 @@ def workload_kb4096(args):
                        "sample": P, "latency_p50_ms": round(plain_lat, 3)},
         "batched_over_plain": round(value / plain_tput, 2),
         "decrypt_failures": failures,
+        "verified_chain": {"gates": verified, "layers": vlayers, "decrypt_failures": vfail},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                      "traffic": traffic},
@@ -402,6 +434,7 @@ def workload_nttbb(args):
                        "latency_p50_ms": round(plain_lat, 3)},
         "batched_over_plain": round(value / plain_tput, 2),
         "decrypt_failures": failures,
+        "verified_chain": {"gates": verified, "layers": vlayers, "decrypt_failures": vfail},
         "bit_exact_sample": bit_exact,
         "roofline": {"bound": "hbm", "achieved": round(key_gbps, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(key_gbps / HBM_PEAK_GBPS, 4),
@@ -512,6 +545,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--verify-gates", type=int, default=1 << 20,
+                    help="kb4096: gates decrypted and checked after timing, as chained layers "
+                         "(0 = off)")
     ap.add_argument("--serve-rates", default="2,5,10", help="circuits/s, one level each")
     ap.add_argument("--serve-circuits", type=int, default=24, help="circuits per rate level")
     ap.add_argument("--serve-pool-gb", type=float, default=4.0)
