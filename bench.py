@@ -128,30 +128,45 @@ def verify_chained(C, plan, dpool, ops, bits, B, k1, total):
     """Decryption check over `total` gates outside the timed region: the
     batch is run as consecutive layers, each layer's outputs becoming the
     next layer's first inputs (second inputs fixed), every output decrypted
-    and compared with the plaintext evaluation. Returns (gates, failures,
-    layers)."""
+    and compared with the plaintext evaluation, and the output phase error
+    (phase - (+-1/8), torus units) accumulated. Returns (gates, failures,
+    layers, noise statistics)."""
     import numpy as np
 
     if total <= 0:
-        return 0, 0, 0
+        return 0, 0, 0, None
     W = C.TLWE1_WORDS
     tt = np.array([[C.TRUTH[o](x, y) for x in (0, 1) for y in (0, 1)] for o in ops], np.uint8)
     xs = bits[:B].astype(np.uint8).copy()
     ys = bits[B:2 * B].astype(np.uint8)
     out = np.empty((B, W), np.uint32)
     done = fails = layers = 0
+    s1 = s2 = 0.0
+    emax = 0.0
+    mu = 0.125  # lvl1 gate output: +-1/8
     while done < total:
         plan.run()
         C.check(C.lib().cufhe_synchronize(), "sync")
         C.check(C.lib().cufhe_memcpy_d2h(C._ptr(out), dpool.at(2 * B * W * 4), out.nbytes, None), "d2h")
         C.check(C.lib().cufhe_stream_synchronize(None), "sync")
         xs = tt[np.arange(B), 2 * xs + ys]
-        fails += int(np.sum(C.decrypt_bits(1, k1, out) != xs))
+        ph = C.phase(1, k1, out)
+        fails += int(np.sum((ph > 0).astype(np.uint8) != xs))
+        err = ph - np.where(xs == 1, mu, -mu)
+        s1 += float(err.sum())
+        s2 += float((err * err).sum())
+        emax = max(emax, float(np.abs(err).max()))
         dpool.upload(out)  # outputs -> wires [0, B): the next layer's first inputs
         done += B
         layers += 1
-    log("chained verification: %d gates over %d layers, %d decryption failures" % (done, layers, fails))
-    return done, fails, layers
+    mean = s1 / done
+    var = s2 / done - mean * mean
+    noise = {"variance": var, "std": var ** 0.5, "mean": mean, "max_abs": emax,
+             "bound": mu, "bound_over_std": mu / var ** 0.5 if var > 0 else None}
+    log("chained verification: %d gates over %d layers, %d decryption failures; output noise "
+        "std %.3g (variance %.3g), max |e| %.3g against the decryption bound 1/8"
+        % (done, layers, fails, noise["std"], var, emax))
+    return done, fails, layers, noise
 
 
 def measured_traffic(workload):
@@ -216,7 +231,8 @@ def workload_kb4096(args):
     log("batched: %.1f gates/s, %.2f ms/step (ks %.2f ms, br %.2f ms)" %
         (value, ms_per_step, statistics.mean(ks_ms), statistics.mean(br_ms)))
     gms = graph_ms(C, gates, dpool, best_c)
-    verified, vfail, vlayers = verify_chained(C, plan, dpool, ops, bits, B, k1, args.verify_gates)
+    verified, vfail, vlayers, vnoise = verify_chained(C, plan, dpool, ops, bits, B, k1,
+                                                      args.verify_gates)
 
     P = min(args.plain_sample, B)
     W = C.TLWE1_WORDS * 4
@@ -269,7 +285,8 @@ def workload_kb4096(args):
                        "sample": P, "latency_p50_ms": round(plain_lat, 3)},
         "batched_over_plain": round(value / plain_tput, 2),
         "decrypt_failures": failures,
-        "verified_chain": {"gates": verified, "layers": vlayers, "decrypt_failures": vfail},
+        "verified_chain": {"gates": verified, "layers": vlayers, "decrypt_failures": vfail,
+                           "output_noise": vnoise},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                      "traffic": traffic},
@@ -434,7 +451,8 @@ def workload_nttbb(args):
                        "latency_p50_ms": round(plain_lat, 3)},
         "batched_over_plain": round(value / plain_tput, 2),
         "decrypt_failures": failures,
-        "verified_chain": {"gates": verified, "layers": vlayers, "decrypt_failures": vfail},
+        "verified_chain": {"gates": verified, "layers": vlayers, "decrypt_failures": vfail,
+                           "output_noise": vnoise},
         "bit_exact_sample": bit_exact,
         "roofline": {"bound": "hbm", "achieved": round(key_gbps, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(key_gbps / HBM_PEAK_GBPS, 4),
