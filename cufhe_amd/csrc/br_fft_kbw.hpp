@@ -57,6 +57,82 @@ struct KbwShared {
     uint32_t bar[C];
 };
 
+// The workgroup's key pipeline as seen by one wave: the slices it loads of
+// every key chunk (t = slot, slot + C, ...; chunk g = (step * l + digit) *
+// (R / RPC) + register group, slice t = (register, (column * 2 + component)
+// * 3 + bundle member)), held in VGPRs PD chunks ahead and written into the
+// two-chunk LDS ring one chunk ahead.
+template <int C>
+struct KbwKeyPipe {
+    using P = lvl1param;
+    static constexpr int N = P::n, H = N / 2, R = H / 64, L = P::l, n0 = lvl0param::n;
+    static constexpr int W = C;
+    static constexpr int RPC = kKbwRpc<C>;
+    static constexpr int NS = 12 * RPC;
+    static constexpr int SPW = (NS + W - 1) / W;
+    static constexpr int PD = CUFHE_KBW_PD;
+    static constexpr int CPD = R / RPC;
+    static constexpr int STEPS = n0 / 2;
+    static constexpr int CHUNKS = STEPS * L * CPD;
+    KbwShared<C>& sm;
+    const int slot, lane;
+    const cplx* bk;
+    cplx kv[PD][SPW];
+    __device__ __forceinline__ KbwKeyPipe(KbwShared<C>& s, int sl, int ln, const cplx* k)
+        : sm(s), slot(sl), lane(ln), bk(k)
+    {
+    }
+    __device__ __forceinline__ void load(int g, cplx (&dst)[SPW])
+    {
+        const int pair = g % CPD, q = (g / CPD) % L, it = g / (CPD * L);
+#pragma unroll
+        for (int k = 0; k < SPW; k++) {
+            const int t = slot + k * W;
+            if ((SPW * W == NS || t < NS) && g < CHUNKS)
+                dst[k] = *kbs_slice_ptr(bk, it, q, RPC * pair + t / 12, t % 12, lane);
+        }
+    }
+    __device__ __forceinline__ void store(int g, const cplx (&src)[SPW])
+    {
+#pragma unroll
+        for (int k = 0; k < SPW; k++) {
+            const int t = slot + k * W;
+            if ((SPW * W == NS || t < NS) && g < CHUNKS)
+                lds_store(&sm.ring[g & 1][t / 12][t % 12][lane], src[k]);
+        }
+    }
+    // before the first barrier: chunk 0 in the ring, chunks 1..PD in flight
+    __device__ __forceinline__ void start()
+    {
+#pragma unroll
+        for (int p = 0; p < PD; p++) load(p, kv[p]);
+        store(0, kv[0]);
+        load(PD, kv[0]);
+    }
+    // after the barrier of chunk g (pair = g % CPD): chunk g+1 into the other
+    // buffer, fetch chunk g+1+PD
+    __device__ __forceinline__ void advance(int g, int pair)
+    {
+        store(g + 1, kv[(pair + 1) % PD]);
+        load(g + 1 + PD, kv[(pair + 1) % PD]);
+    }
+    // A wave without a job: the same loads, stores and barriers as a
+    // rotating wave (one barrier per chunk, nothing else), so a workgroup can
+    // run fewer jobs than waves at the speed of its rotating waves.
+    __device__ __forceinline__ void run_loader_only()
+    {
+        start();
+#pragma unroll 1
+        for (int gq = 0; gq < STEPS * L; gq++) {
+#pragma unroll
+            for (int pair = 0; pair < CPD; pair++) {
+                __syncthreads();  // the rotating waves' chunk barrier
+                advance(gq * CPD + pair, pair);
+            }
+        }
+    }
+};
+
 // The blind rotation of one job (wave `slot` of the workgroup); acc[j] holds
 // component j in AccStore's register order.
 template <int C>
@@ -65,14 +141,9 @@ __device__ __forceinline__ void blind_rotate_kbw(KbwShared<C>& sm, int slot, uin
 {
     using P = lvl1param;
     constexpr int N = P::n, H = N / 2, R = H / 64, L = P::l, n0 = lvl0param::n;
-    constexpr int W = C;                     // waves per workgroup
     constexpr int RPC = kKbwRpc<C>;          // registers per chunk
-    constexpr int NS = 12 * RPC;             // slices per chunk
-    constexpr int SPW = (NS + W - 1) / W;    // slices loaded per wave and chunk
     constexpr int STEPS = n0 / 2;
-    constexpr int PD = CUFHE_KBW_PD;
-    constexpr int CPD = R / RPC;             // chunks per digit pass
-    constexpr int CHUNKS = STEPS * L * CPD;  // chunk g = (it * L + q) * CPD + pair
+    constexpr int CPD = R / RPC;             // chunks per digit pass (g = (it * L + q) * CPD + pair)
     static_assert(R == 8 && P::Bgbit == 8 && L == 2, "lvl1 shapes");
     const int lane = threadIdx.x & 63;
     cplx* const scratch = sm.scratch[slot];
@@ -94,30 +165,8 @@ __device__ __forceinline__ void blind_rotate_kbw(KbwShared<C>& sm, int slot, uin
         acc[1][r] = test_vector_coef<P>(i, bar, mu);
     }
 
-    // Key pipeline: slices of chunk g are held in kv[g % PD] until written.
-    // Slice t of a chunk: register d = RPC * pair + t / 12, ring slice t % 12.
-    cplx kv[PD][SPW];
-    auto load_chunk = [&](int g, cplx (&dst)[SPW]) {
-        const int pair = g % CPD, q = (g / CPD) % L, it = g / (CPD * L);
-#pragma unroll
-        for (int k = 0; k < SPW; k++) {
-            const int t = slot + k * W;
-            if ((SPW * W == NS || t < NS) && g < CHUNKS)
-                dst[k] = *kbs_slice_ptr(bk, it, q, RPC * pair + t / 12, t % 12, lane);
-        }
-    };
-    auto store_chunk = [&](int g, const cplx (&src)[SPW]) {
-#pragma unroll
-        for (int k = 0; k < SPW; k++) {
-            const int t = slot + k * W;
-            if ((SPW * W == NS || t < NS) && g < CHUNKS)
-                lds_store(&sm.ring[g & 1][t / 12][t % 12][lane], src[k]);
-        }
-    };
-#pragma unroll
-    for (int p = 0; p < PD; p++) load_chunk(p, kv[p]);
-    store_chunk(0, kv[0]);
-    load_chunk(PD, kv[0]);
+    KbwKeyPipe<C> pipe(sm, slot, lane, bk);
+    pipe.start();
 
 #pragma unroll 1
     for (int it = 0; it < STEPS; it++) {
@@ -231,9 +280,7 @@ __device__ __forceinline__ void blind_rotate_kbw(KbwShared<C>& sm, int slot, uin
                     // ahead of this block's arithmetic
                     __builtin_amdgcn_sched_barrier(0);
                 }
-                // refill: chunk g+1 into the other buffer, fetch chunk g+1+PD
-                store_chunk(g + 1, kv[(pair + 1) % PD]);
-                load_chunk(g + 1 + PD, kv[(pair + 1) % PD]);
+                pipe.advance(g, pair);
             }
         }
         fft_inverse2<H, 2, true>(accj[0], accj[1], scratch, tables());

@@ -346,6 +346,85 @@ __global__ __launch_bounds__(MUX ? 256 : 128, 2) void k_gate_lvl1_fft(
 }
 
 // ---------------------------------------------------------------------------
+// Plain per-gate form with the key bundle at lvl1 (IKS then BR): one gate per
+// workgroup of eight waves. The gate's rotation (MUX/NMUX: two rotations) runs
+// on wave 0 (and 1) with br_fft_kbw.hpp's one-wave-per-job code; the other
+// waves only stream the key through the LDS ring, so a single gate gets the
+// shared-key pipeline at the latency of one rotating wave. Outputs are
+// bit-identical to k_gate_lvl1_fft (same key switch, same rotation
+// arithmetic).
+template <bool MUX>
+__global__ __launch_bounds__(512, 1) void k_gate_lvl1_kbw(
+    uint32_t* out, const uint32_t* in0, const uint32_t* in1, const uint32_t* inc, int32_t ca,
+    int32_t cb, uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
+    FourierKeys<lvl1param> K)
+{
+    using P = lvl1param;
+    constexpr int C = 8, NJ = MUX ? 2 : 1;
+    constexpr int N = P::n, H = N / 2, R = H / 64, n0 = lvl0param::n;
+    constexpr int NT = 64 * C;
+    constexpr uint32_t mu = static_cast<uint32_t>(P::μ);
+    constexpr uint32_t rnd = kMaskRound<P>;
+    __shared__ KbwShared<C> sm;
+    __shared__ uint32_t sh_tl0[NJ][kTlwe0Pad];
+    __shared__ uint32_t sh_res[MUX ? 2 : 1][MUX ? N + 1 : 1];
+    const int slot = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int t = threadIdx.x;
+    for (int i = t; i < R * 64; i += NT) sm.twl[i] = K.fft.twl[i];
+    for (int i = t; i < 64; i += NT) {
+        sm.tw64[i] = K.fft.tw64[i];
+        sm.e2n_lo[i] = K.e2n[i];
+    }
+    for (int i = t; i < R; i += NT) sm.cr[i] = K.fft.cr[i];
+    for (int i = t; i < 2 * N / 64; i += NT) sm.e2n_hi[i] = K.e2n[64 * i];
+    if constexpr (MUX) {
+        // job 0: c + t - mu ; job 1: -c + f - mu (both AND-like, +-mu out)
+        const int job = t >> 8;
+        key_switch_columns<TFHEpp::lvl10param, false, 3>(sh_tl0[job], inc, job == 0 ? in1 : in0,
+                                                         job == 0 ? 1 : -1, 1, 0u - mu, ksk,
+                                                         t & 255, 256);
+    }
+    else {
+        key_switch_columns<TFHEpp::lvl10param, false, 2>(sh_tl0[0], in0, in1, ca, cb, offset, ksk,
+                                                         t, NT);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int job = 0; job < NJ; job++) {
+        for (int i = t; i <= n0; i += NT) {
+            if (i < n0)
+                sm.abar[job][i] = static_cast<uint16_t>(mod_switch<P>(sh_tl0[job][i] + rnd));
+            else
+                sm.bar[job] = 2 * N - mod_switch<P>(sh_tl0[job][n0]);
+        }
+    }
+    __syncthreads();
+    if (slot < NJ) {
+        uint32_t acc[2][16];
+        blind_rotate_kbw<C>(sm, slot, mu, acc, K.bk);
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            AccStore<P> a;
+#pragma unroll
+            for (int r = 0; r < 16; r++) a.reg[r] = acc[j][r];
+            sample_extract_store<P>(a, j, MUX ? sh_res[slot] : out);
+        }
+    }
+    else {
+        KbwKeyPipe<C> pipe(sm, slot, t & 63, K.bk);
+        pipe.run_loader_only();
+    }
+    if constexpr (MUX) {
+        __syncthreads();
+        for (int i = t; i <= N; i += NT) {
+            uint32_t v = sh_res[0][i] + sh_res[1][i];
+            if (i == N) v += mu;
+            out[i] = mux_sign > 0 ? v : 0u - v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Plain per-gate form, lvl02 API (BR-IKS order): lvl0 in, BR to lvl2
 // (Torus64), SE, key switch lvl2 -> lvl0 (reference src/bootstrap_gpu.cu
 // :1248-1273, :1305-1331, MUX :1949-2060). MUX runs its two rotations one
@@ -548,7 +627,13 @@ void launch_gate_lvl1_fft(bool kb, bool mux, uint32_t* out, const uint32_t* in0,
     hipLaunchKernelGGL((k_gate_lvl1_fft<KBV, MUXV>), grid, block, 0, st, out, in0, in1, inc, \
                        ca, cb, offset, mux_sign, ksk, K)
     if (kb) {
-        if (mux) CUFHE_L(true, true); else CUFHE_L(true, false);
+        // key bundle: the one-wave rotation with seven key-loading waves
+        if (mux)
+            hipLaunchKernelGGL((k_gate_lvl1_kbw<true>), grid, dim3(512), 0, st, out, in0, in1, inc,
+                               ca, cb, offset, mux_sign, ksk, K);
+        else
+            hipLaunchKernelGGL((k_gate_lvl1_kbw<false>), grid, dim3(512), 0, st, out, in0, in1,
+                               inc, ca, cb, offset, mux_sign, ksk, K);
     }
     else {
         if (mux) CUFHE_L(false, true); else CUFHE_L(false, false);
