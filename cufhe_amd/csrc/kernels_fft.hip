@@ -424,6 +424,76 @@ __global__ __launch_bounds__(512, 1) void k_gate_lvl1_kbw(
     }
 }
 
+// Plain per-gate form with the key bundle, lvl0 API (BR then IKS), on the
+// same eight-wave organisation as k_gate_lvl1_kbw: waves 0 (and 1) rotate,
+// the others stream the key; then all 512 threads key-switch. Outputs are
+// bit-identical to k_gate_lvl01_fft.
+template <bool MUX, bool SUBSET>
+__global__ __launch_bounds__(512, 1) void k_gate_lvl01_kbw(
+    uint32_t* out, const uint32_t* in0, const uint32_t* in1, const uint32_t* inc, int32_t ca,
+    int32_t cb, uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
+    FourierKeys<lvl1param> K)
+{
+    using P = lvl1param;
+    constexpr int C = 8, NJ = MUX ? 2 : 1;
+    constexpr int N = P::n, H = N / 2, R = H / 64, n0 = lvl0param::n;
+    constexpr int NT = 64 * C;
+    constexpr uint32_t mu = static_cast<uint32_t>(P::μ);
+    constexpr uint32_t mu0 = static_cast<uint32_t>(lvl0param::μ);
+    constexpr uint32_t rnd = kMaskRound<P>;
+    __shared__ KbwShared<C> sm;
+    __shared__ uint32_t sh_res[NJ][N + 4];
+    const int slot = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int t = threadIdx.x;
+    for (int i = t; i < R * 64; i += NT) sm.twl[i] = K.fft.twl[i];
+    for (int i = t; i < 64; i += NT) {
+        sm.tw64[i] = K.fft.tw64[i];
+        sm.e2n_lo[i] = K.e2n[i];
+    }
+    for (int i = t; i < R; i += NT) sm.cr[i] = K.fft.cr[i];
+    for (int i = t; i < 2 * N / 64; i += NT) sm.e2n_hi[i] = K.e2n[64 * i];
+    // the pre-added lvl0 inputs (as k_gate_lvl01_fft), mod-switched
+#pragma unroll
+    for (int job = 0; job < NJ; job++) {
+        const uint32_t* a = MUX ? inc : in0;
+        const uint32_t* b = MUX ? (job == 0 ? in1 : in0) : in1;
+        const int32_t sa = MUX ? (job == 0 ? 1 : -1) : ca;
+        const int32_t sb = MUX ? 1 : cb;
+        const uint32_t off = MUX ? 0u - mu0 : offset;
+        for (int i = t; i <= n0; i += NT) {
+            uint32_t v = static_cast<uint32_t>(sa) * a[i] + static_cast<uint32_t>(sb) * b[i];
+            if (i < n0)
+                sm.abar[job][i] = static_cast<uint16_t>(mod_switch<P>(v + rnd));
+            else
+                sm.bar[job] = 2 * N - mod_switch<P>(v + off);
+        }
+    }
+    __syncthreads();
+    if (slot < NJ) {
+        uint32_t acc[2][16];
+        blind_rotate_kbw<C>(sm, slot, mu, acc, K.bk);
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            AccStore<P> r;
+#pragma unroll
+            for (int k = 0; k < 16; k++) r.reg[k] = acc[j][k];
+            sample_extract_store<P>(r, j, sh_res[slot]);
+        }
+    }
+    else {
+        KbwKeyPipe<C> pipe(sm, slot, t & 63, K.bk);
+        pipe.run_loader_only();
+    }
+    __syncthreads();
+    if constexpr (MUX)
+        key_switch_columns<TFHEpp::lvl10param, SUBSET, 2>(out, sh_res[0], sh_res[1], mux_sign,
+                                                          mux_sign, mux_sign > 0 ? mu : 0u - mu,
+                                                          ksk, t, NT);
+    else
+        key_switch_columns<TFHEpp::lvl10param, SUBSET, 2>(out, sh_res[0], nullptr, 1, 0, 0u, ksk,
+                                                          t, NT);
+}
+
 // ---------------------------------------------------------------------------
 // Plain per-gate form, lvl02 API (BR-IKS order): lvl0 in, BR to lvl2
 // (Torus64), SE, key switch lvl2 -> lvl0 (reference src/bootstrap_gpu.cu
@@ -650,12 +720,17 @@ void launch_gate_lvl01_fft(bool kb, bool mux, bool subset, uint32_t* out, const 
     hipLaunchKernelGGL((k_gate_lvl01_fft<KBV, MUXV, SUB>), dim3(1), dim3(MUXV ? 256 : 128), 0, \
                        st, out, in0, in1, inc, ca, cb, offset, mux_sign, ksk, K)
     if (kb) {
+        // key bundle: the one-wave rotation with seven key-streaming waves
+#define CUFHE_K01(MUXV, SUB)                                                                     \
+    hipLaunchKernelGGL((k_gate_lvl01_kbw<MUXV, SUB>), dim3(1), dim3(512), 0, st, out, in0, in1, \
+                       inc, ca, cb, offset, mux_sign, ksk, K)
         if (mux) {
-            if (subset) CUFHE_G01(true, true, true); else CUFHE_G01(true, true, false);
+            if (subset) CUFHE_K01(true, true); else CUFHE_K01(true, false);
         }
         else {
-            if (subset) CUFHE_G01(true, false, true); else CUFHE_G01(true, false, false);
+            if (subset) CUFHE_K01(false, true); else CUFHE_K01(false, false);
         }
+#undef CUFHE_K01
     }
     else {
         if (mux) {
