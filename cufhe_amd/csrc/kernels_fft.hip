@@ -377,16 +377,19 @@ __global__ __launch_bounds__(512, 1) void k_gate_lvl1_kbw(
     }
     for (int i = t; i < R; i += NT) sm.cr[i] = K.fft.cr[i];
     for (int i = t; i < 2 * N / 64; i += NT) sm.e2n_hi[i] = K.e2n[64 * i];
+    // key switch over all eight waves (the transform scratch is free yet and
+    // holds the partial sums)
+    uint32_t* const red = reinterpret_cast<uint32_t*>(&sm.scratch[0][0]);
     if constexpr (MUX) {
         // job 0: c + t - mu ; job 1: -c + f - mu (both AND-like, +-mu out)
-        const int job = t >> 8;
-        key_switch_columns<TFHEpp::lvl10param, false, 3>(sh_tl0[job], inc, job == 0 ? in1 : in0,
-                                                         job == 0 ? 1 : -1, 1, 0u - mu, ksk,
-                                                         t & 255, 256);
+        const int job = slot >> 2;
+        key_switch_waves<TFHEpp::lvl10param, false, 4>(
+            sh_tl0[job], inc, job == 0 ? in1 : in0, job == 0 ? 1 : -1, 1, 0u - mu, ksk,
+            red + job * 4 * (n0 + 1), slot & 3, t & 63);
     }
     else {
-        key_switch_columns<TFHEpp::lvl10param, false, 2>(sh_tl0[0], in0, in1, ca, cb, offset, ksk,
-                                                         t, NT);
+        key_switch_waves<TFHEpp::lvl10param, false, 8>(sh_tl0[0], in0, in1, ca, cb, offset, ksk,
+                                                       red, slot, t & 63);
     }
     __syncthreads();
 #pragma unroll
@@ -426,7 +429,7 @@ __global__ __launch_bounds__(512, 1) void k_gate_lvl1_kbw(
 
 // Plain per-gate form with the key bundle, lvl0 API (BR then IKS), on the
 // same eight-wave organisation as k_gate_lvl1_kbw: waves 0 (and 1) rotate,
-// the others stream the key; then all 512 threads key-switch. Outputs are
+// the others stream the key; then all eight waves key-switch. Outputs are
 // bit-identical to k_gate_lvl01_fft.
 template <bool MUX, bool SUBSET>
 __global__ __launch_bounds__(512, 1) void k_gate_lvl01_kbw(
@@ -485,13 +488,15 @@ __global__ __launch_bounds__(512, 1) void k_gate_lvl01_kbw(
         pipe.run_loader_only();
     }
     __syncthreads();
+    // key switch over all eight waves, partial sums in the transform scratch
+    uint32_t* const red = reinterpret_cast<uint32_t*>(&sm.scratch[0][0]);
     if constexpr (MUX)
-        key_switch_columns<TFHEpp::lvl10param, SUBSET, 2>(out, sh_res[0], sh_res[1], mux_sign,
-                                                          mux_sign, mux_sign > 0 ? mu : 0u - mu,
-                                                          ksk, t, NT);
+        key_switch_waves<TFHEpp::lvl10param, SUBSET, 8>(out, sh_res[0], sh_res[1], mux_sign,
+                                                        mux_sign, mux_sign > 0 ? mu : 0u - mu, ksk,
+                                                        red, slot, t & 63);
     else
-        key_switch_columns<TFHEpp::lvl10param, SUBSET, 2>(out, sh_res[0], nullptr, 1, 0, 0u, ksk,
-                                                          t, NT);
+        key_switch_waves<TFHEpp::lvl10param, SUBSET, 8>(out, sh_res[0], nullptr, 1, 0, 0u, ksk,
+                                                        red, slot, t & 63);
 }
 
 // ---------------------------------------------------------------------------

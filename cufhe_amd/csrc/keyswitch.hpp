@@ -110,4 +110,85 @@ __device__ __forceinline__ void key_switch_columns(uint32_t* out, const typename
     }
 }
 
+// The same key switch spread over NW waves of a workgroup for one job (the
+// plain per-gate form, where a single job's key switch is latency-bound):
+// wave w takes switched coefficients j in its 1/NW share, each lane
+// accumulates columns lane + 64 c (c < 10), the partial sums go through
+// red[NW][n0 + 1] in LDS, and threads [0, 64 NW) sum them into out.
+// Integer sums mod 2^32: bit-identical to key_switch_columns. `wave` is the
+// wave's index within the NW waves of this job; every thread of the
+// workgroup must call it (one __syncthreads inside).
+template <class KS, bool SUBSET, int NW>
+__device__ __forceinline__ void key_switch_waves(uint32_t* out, const typename KS::domainP::T* in0,
+                                                 const typename KS::domainP::T* in1, int32_t ca,
+                                                 int32_t cb, typename KS::domainP::T offset,
+                                                 const uint32_t* ksk, uint32_t* red, int wave,
+                                                 int lane)
+{
+    using TR = KsTraits<KS>;
+    using DT = typename TR::DT;
+    constexpr uint32_t first = SUBSET ? TR::n0 : 0;
+    constexpr int NC = (TR::n0 + 1 + 63) / 64;
+    constexpr uint32_t share = (TR::dn - first + NW - 1) / NW;
+    const DT ua = static_cast<DT>(static_cast<std::make_signed_t<DT>>(ca));
+    const DT ub = static_cast<DT>(static_cast<std::make_signed_t<DT>>(cb));
+    uint32_t res[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const int i = lane + 64 * c;
+        uint32_t r = 0;
+        if (wave == 0) {  // base terms once
+            if (i == static_cast<int>(TR::n0)) {
+                DT v = ua * in0[TR::dn] + offset;
+                if (in1) v += ub * in1[TR::dn];
+                r = TR::convert(v);
+            }
+            else if (SUBSET && i < static_cast<int>(TR::n0)) {
+                DT v = ua * in0[i];
+                if (in1) v += ub * in1[i];
+                r = TR::convert(v);
+            }
+        }
+        res[c] = r;
+    }
+    const uint32_t j0 = first + static_cast<uint32_t>(wave) * share;
+    const uint32_t j1 = j0 + share < TR::dn ? j0 + share : TR::dn;
+#pragma unroll 4
+    for (uint32_t j = j0; j < j1; j++) {
+        DT tmp = ua * in0[j] + TR::kOffset;
+        if (in1) tmp += ub * in1[j];
+        const uint32_t* kj = ksk + static_cast<size_t>(j - first) * TR::coef_words;
+#pragma unroll
+        for (uint32_t k = 0; k < KS::t; k++) {
+            const int32_t val =
+                static_cast<int32_t>((tmp >> (TR::W - (k + 1) * KS::basebit)) &
+                                     ((static_cast<DT>(1) << KS::basebit) - 1)) -
+                static_cast<int32_t>(TR::numbase);
+            if (val == 0) continue;
+            const uint32_t* row = kj + k * TR::digit_words + (abs(val) - 1) * TR::row_words;
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const int i = lane + 64 * c;
+                if (i <= static_cast<int>(TR::n0)) {
+                    const uint32_t x = row[i];
+                    res[c] = val > 0 ? res[c] - x : res[c] + x;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const int i = lane + 64 * c;
+        if (i <= static_cast<int>(TR::n0)) red[wave * (TR::n0 + 1) + i] = res[c];
+    }
+    __syncthreads();
+    const int tid = wave * 64 + lane;
+    for (int i = tid; i <= static_cast<int>(TR::n0); i += 64 * NW) {
+        uint32_t r = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) r += red[w * (TR::n0 + 1) + i];
+        out[i] = r;
+    }
+}
+
 }  // namespace cufhe_amd
