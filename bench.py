@@ -424,6 +424,8 @@ def workload_nttbb(args):
     total, step_ms, ks_ms, br_ms, _ = timed_steps(C, plan, args.steps)
     value = B * args.steps / total
     gms = graph_ms(C, gates, dpool, best_c)
+    verified, vfail, vlayers, vnoise = verify_chained(C, plan, dpool, ops, bits, B, k1,
+                                                      args.verify_gates)
     W = C.TLWE1_WORDS * 4
 
     def gate_fn(i, st):
@@ -564,8 +566,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--verify-gates", type=int, default=1 << 20,
-                    help="kb4096: gates decrypted and checked after timing, as chained layers "
-                         "(0 = off)")
+                    help="kb4096, nttbb: gates decrypted and checked after timing, as chained "
+                         "layers (0 = off)")
     ap.add_argument("--serve-rates", default="2,5,10", help="circuits/s, one level each")
     ap.add_argument("--serve-circuits", type=int, default=24, help="circuits per rate level")
     ap.add_argument("--serve-pool-gb", type=float, default=4.0)
