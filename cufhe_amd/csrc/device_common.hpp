@@ -101,6 +101,31 @@ struct BrJob {
 constexpr uint32_t kBrOutTrlwe = 1;
 static_assert(sizeof(BrJob) == 40, "BrJob layout");
 
+// Compute units of the current device (queried once per device).
+inline int device_cu_count()
+{
+    static int cached[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int v = __atomic_load_n(&cached[dev], __ATOMIC_RELAXED);
+    if (v <= 0) {
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            v <= 0)
+            v = 256;
+        __atomic_store_n(&cached[dev], v, __ATOMIC_RELAXED);
+    }
+    return v;
+}
+
+// Grid for workgroups of up to c jobs with the jobs spread evenly: the fewest
+// whole waves of CUs that hold njobs (never more workgroups than jobs).
+inline int spread_grid(int njobs, int c)
+{
+    const int cus = device_cu_count();
+    const int rounds = (njobs + c * cus - 1) / (c * cus);
+    return njobs < rounds * cus ? njobs : rounds * cus;
+}
+
 // One key-switch job: out_lvl0 = KS(ca*in0 + cb*in1 (+offset on b)).
 struct KsJob {
     const void* in0;

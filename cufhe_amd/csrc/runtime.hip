@@ -490,22 +490,6 @@ void lvl1_build(const cufhe_gate_desc* gates, int n, uint32_t* d_pool, uint32_t*
 
 bool lvl1_loaded() { return state().k1.loaded; }
 
-// Compute units of the current device (queried once per device).
-static int device_cu_count()
-{
-    static int cached[64] = {};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-    int v = __atomic_load_n(&cached[dev], __ATOMIC_RELAXED);
-    if (v <= 0) {
-        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            v <= 0)
-            v = 256;
-        __atomic_store_n(&cached[dev], v, __ATOMIC_RELAXED);
-    }
-    return v;
-}
-
 int lvl1_pick_c(int c, int nbr)
 {
     const LevelKeys<lvl1param>& k = state().k1;
@@ -517,20 +501,12 @@ int lvl1_pick_c(int c, int nbr)
             c = 2;
         }
         else if (k.kb) {
-            // The shared-key kernels run one workgroup per CU and take a
-            // near-constant time per round of workgroups (MI355X, per round:
-            // 6 jobs/CU in 5.9 ms, 8 jobs/CU in 6.3 ms; the one-job kernel
-            // needs 7.5 ms even for a single round), so pick the one with
-            // the shorter rounds x time: 6 up to 1536 jobs and at 3072, 8 at
-            // 2048, 4096 and beyond. nbr < 0 (size unknown): 8.
-            if (nbr < 0) {
-                c = 8;
-            }
-            else {
-                const long cus = device_cu_count();
-                const long r6 = (nbr + 6 * cus - 1) / (6 * cus), r8 = (nbr + 8 * cus - 1) / (8 * cus);
-                c = r6 * 59 < r8 * 63 ? 6 : 8;
-            }
+            // The one-wave-per-job kernel spreads the jobs over all CUs (at
+            // most 8 per workgroup; waves without a job only stream the key)
+            // and is the fastest at every batch size (MI355X, BR ms, C = 6 /
+            // 8): 256 jobs 5.84 / 3.54, 1024: 5.84 / 4.27, 1536: 6.19 / 5.55,
+            // 2048: 12.05 / 6.41, 3072: 11.73 / 11.11, 4096: 17.71 / 12.41.
+            c = 8;
         }
         else {
             c = 1;

@@ -157,9 +157,8 @@ int cufhe_gate_lvl02(int op, uint32_t* out, const uint32_t* in0, const uint32_t*
  * jobs_per_group: bootstraps per workgroup (1, 2, 4, 6, 8; 0 = default).
  * With the key-bundle keys, 6 and 8 are the kernels that share each key chunk
  * among the workgroup's jobs through LDS (6: two waves per job, 8: one wave
- * per job; the default picks whichever needs the shorter rounds of
- * workgroups for the batch); with the NTT keys, 6 is the shared-key NTT
- * kernel. use_graph != 0 captures the launches in a HIP graph
+ * per job with the jobs spread over all CUs, the default); with the NTT
+ * keys, 6 is the shared-key NTT kernel. use_graph != 0 captures the launches in a HIP graph
  * on the first run and replays it. */
 int cufhe_plan_create_lvl1(const cufhe_gate_desc* gates, int n, uint32_t* d_pool,
                            int jobs_per_group, int use_graph, cufhe_plan** plan);
