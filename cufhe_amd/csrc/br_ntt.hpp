@@ -150,28 +150,32 @@ __device__ __forceinline__ void br_step_ntt(uint32_t (&acc)[16], int j, const ui
         for (int d = 0; d < 16; d++) {
             if (d + 1 < 16) load_keys((d + 1) & 1, d + 1);
             const int sl = d & 1;
-            uint32_t acc_d = s[d];
+            // products summed in pairs before one Montgomery reduction
+            // (mont_dot2): the same residues as reducing every product
+            uint32_t D[2], kc[2];
 #pragma unroll
             for (int comp = 0; comp < 2; comp++) {
-                const uint32_t D = (comp == j ? my_lds : partner_lds)[d * 64 + lane];
+                D[comp] = (comp == j ? my_lds : partner_lds)[d * 64 + lane];
                 uint32_t k;
                 if constexpr (MODE == NTT_PLAIN) {
                     k = kv[sl][comp][0];
                 }
                 else if constexpr (MODE == NTT_KB) {
                     k = comp == j ? one_q : 0u;
-                    k = mod_add(k, mont_mul(xg[1].at(d), kv[sl][comp][2]));
-                    k = mod_add(k, mont_mul(xg[0].at(d), kv[sl][comp][1]));
+                    k = mod_add(k, mont_dot2(xg[1].at(d), kv[sl][comp][2], xg[0].at(d), kv[sl][comp][1]));
                     k = mod_add(k, mont_mul(xg[2].at(d), kv[sl][comp][0]));
                 }
                 else {
                     k = 0;
 #pragma unroll
-                    for (int e = 0; e < NK; e++) k = mod_add(k, mont_mul(xg[e].at(d), kv[sl][comp][e]));
+                    for (int e = 0; e + 1 < NK; e += 2)
+                        k = mod_add(k, mont_dot2(xg[e].at(d), kv[sl][comp][e], xg[e + 1].at(d),
+                                                 kv[sl][comp][e + 1]));
+                    if constexpr (NK % 2) k = mod_add(k, mont_mul(xg[NK - 1].at(d), kv[sl][comp][NK - 1]));
                 }
-                acc_d = mod_add(acc_d, mont_mul(D, k));
+                kc[comp] = k;
             }
-            s[d] = acc_d;
+            s[d] = mod_add(s[d], mont_dot2(D[0], kc[0], D[1], kc[1]));
         }
         __syncthreads();
     }

@@ -189,10 +189,11 @@ __device__ __forceinline__ void blind_rotate_ntt_shared(NttShared<C>& sm, int sl
 #pragma unroll
                 for (int dd = 0; dd < 4; dd++) {
                     const int d = c4 * 4 + dd;
-                    uint32_t acc_d = s[d];
+                    // products summed in pairs before one reduction (mont_dot2)
+                    uint32_t D[2], kc[2];
 #pragma unroll
                     for (int comp = 0; comp < 2; comp++) {
-                        const uint32_t D = (comp == j ? my_lds : partner_lds)[d * 64 + lane];
+                        D[comp] = (comp == j ? my_lds : partner_lds)[d * 64 + lane];
                         const int s0 = (j * 2 + comp) * NK;
                         uint32_t k;
                         if constexpr (MODE == NTT_PLAIN) {
@@ -200,18 +201,24 @@ __device__ __forceinline__ void blind_rotate_ntt_shared(NttShared<C>& sm, int sl
                         }
                         else if constexpr (MODE == NTT_KB) {
                             k = comp == j ? one_q : 0u;
-                            k = mod_add(k, mont_mul(xg[1].at(d), ring[s0 + 2][dd * 64 + lane]));
-                            k = mod_add(k, mont_mul(xg[0].at(d), ring[s0 + 1][dd * 64 + lane]));
+                            k = mod_add(k, mont_dot2(xg[1].at(d), ring[s0 + 2][dd * 64 + lane],
+                                                     xg[0].at(d), ring[s0 + 1][dd * 64 + lane]));
                             k = mod_add(k, mont_mul(xg[2].at(d), ring[s0 + 0][dd * 64 + lane]));
                         }
                         else {
                             k = 0;
 #pragma unroll
-                            for (int e = 0; e < NK; e++)
-                                k = mod_add(k, mont_mul(xg[e].at(d), ring[s0 + e][dd * 64 + lane]));
+                            for (int e = 0; e + 1 < NK; e += 2)
+                                k = mod_add(k, mont_dot2(xg[e].at(d), ring[s0 + e][dd * 64 + lane],
+                                                         xg[e + 1].at(d),
+                                                         ring[s0 + e + 1][dd * 64 + lane]));
+                            if constexpr (NK % 2)
+                                k = mod_add(k, mont_mul(xg[NK - 1].at(d),
+                                                        ring[s0 + NK - 1][dd * 64 + lane]));
                         }
-                        acc_d = mod_add(acc_d, mont_mul(D, k));
+                        kc[comp] = k;
                     }
+                    uint32_t acc_d = mod_add(s[d], mont_dot2(D[0], kc[0], D[1], kc[1]));
                     // keep this chunk's arithmetic ahead of the next barrier
                     asm volatile("" : "+v"(acc_d));
                     s[d] = acc_d;

@@ -94,6 +94,19 @@ __device__ __forceinline__ uint32_t mont_mul(uint32_t a, uint32_t b)
     return r >= ntt31::P ? r - ntt31::P : r;
 }
 
+// (a1 b1 + a2 b2) R^-1 mod P with one Montgomery reduction: the 64-bit sum
+// is below 2 P^2, so t + m P < 2^64 and the result is below 2P before the
+// final subtraction. Equal to mod_add(mont_mul(a1, b1), mont_mul(a2, b2))
+// for canonical inputs (every value here is a residue in [0, P)).
+__device__ __forceinline__ uint32_t mont_dot2(uint32_t a1, uint32_t b1, uint32_t a2, uint32_t b2)
+{
+    const uint64_t t = static_cast<uint64_t>(a1) * b1 + static_cast<uint64_t>(a2) * b2;
+    const uint32_t m = static_cast<uint32_t>(t) * ntt31::PINV;
+    const uint64_t u = t + static_cast<uint64_t>(m) * ntt31::P;
+    const uint32_t r = static_cast<uint32_t>(u >> 32);
+    return r >= ntt31::P ? r - ntt31::P : r;
+}
+
 // Multiply by the compile-time twiddle W_M^E (forward: omega_M^E, inverse:
 // omega_M^-E; omega_M = psi^(2048/M)), data in normal form.
 template <int M, int E, bool INV>
