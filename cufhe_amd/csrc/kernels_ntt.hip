@@ -225,30 +225,36 @@ __global__ __launch_bounds__(MUX ? 256 : 128, 2) void k_gate_lvl01_ntt(
 // Plain per-gate form, lvl1 API (IKS-BR order), one gate per launch; MUX runs
 // its two rotations on two wave pairs and combines them.
 template <int MODE, bool SUBSET, bool MUX>
-__global__ __launch_bounds__(MUX ? 256 : 128, 2) void k_gate_lvl1_ntt(
+__global__ __launch_bounds__(512, 1) void k_gate_lvl1_ntt(
     uint32_t* out, const uint32_t* in0, const uint32_t* in1, const uint32_t* inc, int32_t ca,
     int32_t cb, uint32_t offset, int32_t mux_sign, const uint32_t* ksk, NttKeys K)
 {
     using P = lvl1param;
     constexpr int C = MUX ? 2 : 1;
-    constexpr int N = P::n;
+    constexpr int N = P::n, n0 = lvl0param::n;
     constexpr uint32_t mu = static_cast<uint32_t>(P::μ);
     __shared__ uint32_t sh_lds[2 * C][kNttLdsWords];
     __shared__ uint32_t sh_tl0[C][kNttTlwe0Pad];
     __shared__ uint32_t sh_res[MUX ? 2 : 1][MUX ? N + 1 : 1];
+    __shared__ uint32_t sh_red[8 * (n0 + 1)];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int slot = wave >> 1, j = wave & 1;
-    const int t = threadIdx.x & 127;
+    // key switch over all eight waves of the workgroup (a single gate's key
+    // switch is a long dependent loop), then only the rotation's 2 C waves
+    // go on: the others end here, and a workgroup barrier waits only for
+    // the waves that have not ended
     if constexpr (MUX) {
-        const uint32_t* b = slot == 0 ? in1 : in0;
-        key_switch_columns<TFHEpp::lvl10param, SUBSET, 5>(sh_tl0[slot], inc, b, slot == 0 ? 1 : -1,
-                                                          1, 0u - mu, ksk, t, 128);
+        const int job = wave >> 2;
+        key_switch_waves<TFHEpp::lvl10param, SUBSET, 4>(
+            sh_tl0[job], inc, job == 0 ? in1 : in0, job == 0 ? 1 : -1, 1, 0u - mu, ksk,
+            sh_red + job * 4 * (n0 + 1), wave & 3, threadIdx.x & 63);
     }
     else {
-        key_switch_columns<TFHEpp::lvl10param, SUBSET, 5>(sh_tl0[0], in0, in1, ca, cb, offset, ksk,
-                                                          t, 128);
+        key_switch_waves<TFHEpp::lvl10param, SUBSET, 8>(sh_tl0[0], in0, in1, ca, cb, offset, ksk,
+                                                        sh_red, wave, threadIdx.x & 63);
     }
     __syncthreads();
+    if (wave >= 2 * C) return;
     uint32_t acc[16];
     blind_rotate_ntt<MODE>(sh_tl0[slot], mu, acc, j, sh_lds[wave], sh_lds[wave ^ 1], K);
     if constexpr (!MUX) {
@@ -331,10 +337,10 @@ static void launch_gate_ntt_mode(bool mux, uint32_t* out, const uint32_t* in0, c
                                  hipStream_t st)
 {
     if (mux)
-        hipLaunchKernelGGL((k_gate_lvl1_ntt<MODE, SUBSET, true>), dim3(1), dim3(256), 0, st, out,
+        hipLaunchKernelGGL((k_gate_lvl1_ntt<MODE, SUBSET, true>), dim3(1), dim3(512), 0, st, out,
                            in0, in1, inc, ca, cb, offset, mux_sign, ksk, K);
     else
-        hipLaunchKernelGGL((k_gate_lvl1_ntt<MODE, SUBSET, false>), dim3(1), dim3(128), 0, st, out,
+        hipLaunchKernelGGL((k_gate_lvl1_ntt<MODE, SUBSET, false>), dim3(1), dim3(512), 0, st, out,
                            in0, in1, inc, ca, cb, offset, mux_sign, ksk, K);
 }
 
