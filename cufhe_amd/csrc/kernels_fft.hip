@@ -94,7 +94,11 @@ __global__ __launch_bounds__(256) void k_from_fourier(double* out, const cplx* i
 }
 
 // ---------------------------------------------------------------------------
-// Batched blind rotation: C jobs per workgroup, two waves per job.
+// Batched blind rotation: C jobs per workgroup, two waves per job. The
+// workgroups walk the job groups with a grid stride, so a grid that fits on
+// the device at once keeps every workgroup on the same rotation step: their
+// key reads then meet in L2 instead of drifting apart as later workgroups
+// start when earlier ones finish.
 template <class P, bool KB, int C>
 __global__ __launch_bounds__(128 * C, CUFHE_BR_WAVES_PER_SIMD) void k_br_fft(const BrJob* __restrict__ jobs, int njobs,
                                                      typename P::T mu, FourierKeys<P> K)
@@ -108,25 +112,49 @@ __global__ __launch_bounds__(128 * C, CUFHE_BR_WAVES_PER_SIMD) void k_br_fft(con
     __shared__ T sh_coef[kL ? 2 * C : 1][kL ? N : 1];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int slot = wave >> 1, j = wave & 1;
-    const int raw = blockIdx.x * C + slot;
-    const bool live = raw < njobs;
-    const BrJob jb = jobs[live ? raw : njobs - 1];
-    for (int i = threadIdx.x & 127; i <= n0; i += 128) {
-        uint32_t v = static_cast<uint32_t>(jb.ca) * jb.in0[i];
-        if (jb.in1) v += static_cast<uint32_t>(jb.cb) * jb.in1[i];
-        if (i == n0) v += jb.offset;
-        sh_tl0[slot][i] = v;
+    const int ngroups = (njobs + C - 1) / C;
+    for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+        const int raw = grp * C + slot;
+        const bool live = raw < njobs;
+        const BrJob jb = jobs[live ? raw : njobs - 1];
+        // the previous group's rotation has finished reading sh_tl0
+        __syncthreads();
+        for (int i = threadIdx.x & 127; i <= n0; i += 128) {
+            uint32_t v = static_cast<uint32_t>(jb.ca) * jb.in0[i];
+            if (jb.in1) v += static_cast<uint32_t>(jb.cb) * jb.in1[i];
+            if (i == n0) v += jb.offset;
+            sh_tl0[slot][i] = v;
+        }
+        __syncthreads();
+        AccStore<P> acc;
+        if constexpr (kL) acc.lds = sh_coef[wave];
+        blind_rotate_fft<P, KB>(sh_tl0[slot], mu, acc, j, sh_spec[wave], sh_spec[wave ^ 1], K);
+        if (live) {
+            if (jb.flags & kBrOutTrlwe)
+                trlwe_store<P>(acc, j, static_cast<T*>(jb.out));
+            else
+                sample_extract_store<P>(acc, j, static_cast<T*>(jb.out));
+        }
     }
-    __syncthreads();
-    AccStore<P> acc;
-    if constexpr (kL) acc.lds = sh_coef[wave];
-    blind_rotate_fft<P, KB>(sh_tl0[slot], mu, acc, j, sh_spec[wave], sh_spec[wave ^ 1], K);
-    if (live) {
-        if (jb.flags & kBrOutTrlwe)
-            trlwe_store<P>(acc, j, static_cast<T*>(jb.out));
-        else
-            sample_extract_store<P>(acc, j, static_cast<T*>(jb.out));
+}
+
+// Workgroups of k_br_fft<P, KB, C> resident on the device at once (0 if the
+// runtime cannot tell), cached per instance.
+template <class P, bool KB, int C>
+static int br_fft_resident_groups()
+{
+    static int cached = -1;
+    int v = __atomic_load_n(&cached, __ATOMIC_RELAXED);
+    if (v < 0) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_br_fft<P, KB, C>, 128 * C, 0) !=
+                hipSuccess ||
+            per_cu < 0)
+            per_cu = 0;
+        v = per_cu * device_cu_count();
+        __atomic_store_n(&cached, v, __ATOMIC_RELAXED);
     }
+    return v;
 }
 
 // ---------------------------------------------------------------------------
@@ -667,9 +695,18 @@ void launch_br_fft(const BrJob* jobs, int njobs, int c, typename P::T mu,
     const dim3 block(128 * c);
     const dim3 grid((njobs + c - 1) / c);
     switch (c) {
-        case 1:
-            hipLaunchKernelGGL((k_br_fft<P, KB, 1>), grid, block, 0, st, jobs, njobs, mu, K);
+        case 1: {
+            // one resident grid (grid-stride over the jobs) for the Torus64
+            // rotation, whose key (167 MB, KB 250 MB) outgrows the caches
+            // once the workgroups drift apart
+            dim3 g1 = grid;
+            if constexpr (sizeof(typename P::T) == 8) {
+                const int cap = br_fft_resident_groups<P, KB, 1>();
+                if (cap > 0 && static_cast<int>(g1.x) > cap) g1.x = cap;
+            }
+            hipLaunchKernelGGL((k_br_fft<P, KB, 1>), g1, block, 0, st, jobs, njobs, mu, K);
             break;
+        }
         case 2:
             if constexpr (sizeof(typename P::T) == 4)
                 hipLaunchKernelGGL((k_br_fft<P, KB, 2>), grid, block, 0, st, jobs, njobs, mu,

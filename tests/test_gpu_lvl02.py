@@ -118,3 +118,29 @@ def test_lvl02_rotation_phase_matches_exact_oracle(lvl02):
         print("lvl2 phase gpu", pg, "exact", pe)
         assert abs(abs(pe) - 0.125) < 2 ** -16
         assert abs(pg - pe) < 2 ** -20
+
+
+def test_lvl02_batch_larger_than_resident_grid(lvl02):
+    """1100 lvl2 rotations in one launch: more jobs than the workgroups that
+    fit on the device at once, so the grid walks them with a stride. Every
+    output must equal the same job's output from a launch small enough for
+    one pass, bit for bit, and carry a test-vector phase of +-1/8."""
+    k2 = lvl02["k2"]
+    rng = np.random.default_rng(203)
+    n = 1100
+    tl0 = rng.integers(0, 2**32, (n, W0), dtype=np.uint64).astype(np.uint32)
+    mu2 = 1 << 61
+    din = C.DeviceBuffer.from_numpy(tl0)
+    dout = C.DeviceBuffer(n * 2049 * 8)
+    C.check(C.lib().cufhe_bootstrap_lvl02(dout.ptr, din.ptr, mu2, n, None), "bootstrap_lvl02")
+    big = dout.download(np.uint64, (n, 2049))
+    pick = np.r_[0:8, 520:528, n - 8:n]
+    sub = C.DeviceBuffer.from_numpy(np.ascontiguousarray(tl0[pick]))
+    sout = C.DeviceBuffer(len(pick) * 2049 * 8)
+    C.check(C.lib().cufhe_bootstrap_lvl02(sout.ptr, sub.ptr, mu2, len(pick), None),
+            "bootstrap_lvl02")
+    small = sout.download(np.uint64, (len(pick), 2049))
+    assert np.array_equal(big[pick], small)
+    ph = np.abs(C.phase(2, k2, big))
+    print("lvl2 |phase| range", ph.min(), ph.max())
+    assert np.all(np.abs(ph - 0.125) < 2 ** -16)
