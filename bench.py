@@ -376,7 +376,7 @@ def workload_lvl2mix(args):
                                "mux": round(muxr["gates_per_s"] / pm, 2)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": None},
+                     "traffic": measured_traffic("lvl2mix")},
         "cpu_baseline": None,
     })
     return res
