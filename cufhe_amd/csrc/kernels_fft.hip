@@ -342,34 +342,38 @@ __global__ __launch_bounds__(256) void k_se_ks_lvl10(uint32_t* out, const uint32
 // Binary gates: one job; MUX/NMUX: two jobs run concurrently, then combined
 // (reference src/bootstrap_gpu.cu:1223-1246, :2069-2197).
 template <bool KB, bool MUX>
-__global__ __launch_bounds__(MUX ? 256 : 128, 2) void k_gate_lvl1_fft(
+__global__ __launch_bounds__(512, 1) void k_gate_lvl1_fft(
     uint32_t* out, const uint32_t* in0, const uint32_t* in1, const uint32_t* inc, int32_t ca,
     int32_t cb, uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
     FourierKeys<lvl1param> K)
 {
     using P = lvl1param;
     constexpr int C = MUX ? 2 : 1;
-    constexpr int N = P::n, H = N / 2;
+    constexpr int N = P::n, H = N / 2, n0 = lvl0param::n;
     constexpr uint32_t mu = static_cast<uint32_t>(P::μ);
     constexpr int NB = BrShape<P>::NB;
     __shared__ cplx sh_spec[2 * C][NB * H];
     __shared__ uint32_t sh_tl0[C][kTlwe0Pad];
     __shared__ uint32_t sh_res[MUX ? 2 : 1][MUX ? N + 1 : 1];
+    __shared__ uint32_t sh_red[8 * (n0 + 1)];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int slot = wave >> 1, j = wave & 1;
-    const int t = threadIdx.x & 127;
+    // key switch over all eight waves of the workgroup (MUX: four per
+    // rotation input), then only the rotation's 2 C waves go on; a
+    // workgroup barrier waits only for the waves that have not ended
     if constexpr (MUX) {
-        // slot 0: c + t - mu ; slot 1: -c + f - mu (both AND-like, +-mu out)
-        const uint32_t* b = slot == 0 ? in1 : in0;
-        key_switch_columns<TFHEpp::lvl10param, false, 5>(sh_tl0[slot], inc, b,
-                                                         slot == 0 ? 1 : -1, 1, 0u - mu, ksk,
-                                                         t, 128);
+        // job 0: c + t - mu ; job 1: -c + f - mu (both AND-like, +-mu out)
+        const int job = wave >> 2;
+        key_switch_waves<TFHEpp::lvl10param, false, 4>(
+            sh_tl0[job], inc, job == 0 ? in1 : in0, job == 0 ? 1 : -1, 1, 0u - mu, ksk,
+            sh_red + job * 4 * (n0 + 1), wave & 3, threadIdx.x & 63);
     }
     else {
-        key_switch_columns<TFHEpp::lvl10param, false, 5>(sh_tl0[0], in0, in1, ca, cb, offset,
-                                                         ksk, t, 128);
+        key_switch_waves<TFHEpp::lvl10param, false, 8>(sh_tl0[0], in0, in1, ca, cb, offset, ksk,
+                                                       sh_red, wave, threadIdx.x & 63);
     }
     __syncthreads();
+    if (wave >= 2 * C) return;
     AccStore<P> acc;
     blind_rotate_fft<P, KB>(sh_tl0[slot], mu, acc, j, sh_spec[wave], sh_spec[wave ^ 1], K);
     if constexpr (!MUX) {
@@ -748,7 +752,7 @@ void launch_gate_lvl1_fft(bool kb, bool mux, uint32_t* out, const uint32_t* in0,
                           uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
                           const FourierKeys<lvl1param>& K, hipStream_t st)
 {
-    const dim3 grid(1), block(mux ? 256 : 128);
+    const dim3 grid(1), block(512);
 #define CUFHE_L(KBV, MUXV)                                                                    \
     hipLaunchKernelGGL((k_gate_lvl1_fft<KBV, MUXV>), grid, block, 0, st, out, in0, in1, inc, \
                        ca, cb, offset, mux_sign, ksk, K)
