@@ -262,8 +262,9 @@ def test_plain_gates_truth_tables(engine, keys_lvl1, kb):
         assert eq >= 0.999
 
 
-def test_plain_mux_nmux_and_not_copy(engine, keys_lvl1):
-    load_lvl1(keys_lvl1, True)
+@pytest.mark.parametrize("kb", [True, False])
+def test_plain_mux_nmux_and_not_copy(engine, keys_lvl1, kb):
+    load_lvl1(keys_lvl1, kb)
     ops, a, b, c = [], [], [], []
     for nm in ("MUX", "NMUX"):
         for s in (0, 1):
@@ -293,8 +294,8 @@ def test_plain_mux_nmux_and_not_copy(engine, keys_lvl1):
         else:
             want.append(x)
     assert np.array_equal(dec, np.array(want, np.uint8))
-    e = O.gate_lvl1(O.EXACT, True, C.GATES["MUX"], ca[5], cb[5], cc[5], keys_lvl1["bk_kb"],
-                    keys_lvl1["ksk"])
+    bk = keys_lvl1["bk_kb"] if kb else keys_lvl1["bk"]
+    e = O.gate_lvl1(O.EXACT, kb, C.GATES["MUX"], ca[5], cb[5], cc[5], bk, keys_lvl1["ksk"])
     assert np.mean(out[5] == e) >= 0.999
 
 
@@ -335,6 +336,47 @@ def test_batch_plan_matches_plain_form(engine, keys_lvl1):
             want.append(1 - (y if s else x))
         elif o == "NOT":
             want.append(1 - x)
+        else:
+            want.append(C.TRUTH[o](x, y))
+    assert np.array_equal(dec, np.array(want, np.uint8))
+
+
+def test_cmux_batch_plan_matches_plain_form(engine, keys_lvl1):
+    """Plain CMUX keys (636 steps): batched plans with 1, 2, 4 and 8 jobs per
+    workgroup, eager and graph, equal the plain per-gate form (eight-wave key
+    switch, then the two rotating waves) bit for bit, MUX/NMUX included."""
+    load_lvl1(keys_lvl1, False)
+    rng = np.random.default_rng(56)
+    n = 40
+    names = ["NAND", "XOR", "OR", "MUX", "NMUX", "ANDYN"]
+    ops = [names[i] for i in rng.integers(0, len(names), n)]
+    bits = rng.integers(0, 2, 3 * n).astype(np.uint8)
+    pool_ct = C.encrypt_bits(1, 901, keys_lvl1["k1"], bits)
+    pool = np.zeros((4 * n, W1), np.uint32)
+    pool[: 3 * n] = pool_ct
+    gates = [(ops[i], 3 * n + i, i, n + i, 2 * n + i) for i in range(n)]
+    res = {}
+    for c, graph in [(1, False), (2, False), (4, False), (8, False), (2, True)]:
+        dpool = C.DeviceBuffer.from_numpy(pool)
+        plan = C.Plan(gates, dpool, jobs_per_group=c, use_graph=graph)
+        for _ in range(2 if graph else 1):
+            plan.run()
+            C.check(C.lib().cufhe_synchronize(), "sync")
+        res[(c, graph)] = dpool.download(np.uint32, pool.shape)[3 * n:]
+        plan.destroy()
+    base = res[(1, False)]
+    for key, v in res.items():
+        assert np.array_equal(v, base), key
+    _, _, _, plain = _run_gates_plain_pool(keys_lvl1, ops, pool_ct, n)
+    assert np.array_equal(plain, base)
+    dec = C.decrypt_bits(1, keys_lvl1["k1"], base)
+    want = []
+    for i, o in enumerate(ops):
+        x, y, s = int(bits[i]), int(bits[n + i]), int(bits[2 * n + i])
+        if o == "MUX":
+            want.append(y if s else x)
+        elif o == "NMUX":
+            want.append(1 - (y if s else x))
         else:
             want.append(C.TRUTH[o](x, y))
     assert np.array_equal(dec, np.array(want, np.uint8))
