@@ -351,6 +351,21 @@ __device__ __forceinline__ void blind_rotate_fft(const uint32_t* tl0, typename P
     }
 }
 
+// Workgroup barriers of one blind_rotate_fft<P, KB>: br_step_fft calls
+// __syncthreads() twice per digit pass, and blind_rotate_fft none of its
+// own. A wave of the workgroup that takes no part in the rotation calls
+// rotation_barriers<P, KB>() in its place to stay in step with the
+// rotating waves (every barrier counts all waves that have not ended).
+template <class P, bool KB>
+inline constexpr int kBrBarriers = (KB ? lvl0param::n / 2 : lvl0param::n) * BrShape<P>::kPasses * 2;
+
+template <class P, bool KB>
+__device__ __forceinline__ void rotation_barriers()
+{
+#pragma unroll 1
+    for (int i = 0; i < kBrBarriers<P, KB>; i++) __syncthreads();
+}
+
 // The rotated TRLWE itself: component j's N coefficients at out + j N.
 template <class P>
 __device__ __forceinline__ void trlwe_store(const AccStore<P>& acc, int j, typename P::T* out)

@@ -274,7 +274,7 @@ __global__ __launch_bounds__(64 * C, 8 / C) void k_br_fft_kbw(const BrJob* __res
 // MUX/NMUX rotate c + t - mu and -c + f - mu on two wave pairs and key-switch
 // +-(r1 + r0 + mu) in one pass.
 template <bool KB, bool MUX, bool SUBSET>
-__global__ __launch_bounds__(MUX ? 256 : 128, 2) void k_gate_lvl01_fft(
+__global__ __launch_bounds__(512, 1) void k_gate_lvl01_fft(
     uint32_t* out, const uint32_t* in0, const uint32_t* in1, const uint32_t* inc, int32_t ca,
     int32_t cb, uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
     FourierKeys<lvl1param> K)
@@ -288,10 +288,14 @@ __global__ __launch_bounds__(MUX ? 256 : 128, 2) void k_gate_lvl01_fft(
     __shared__ cplx sh_spec[2 * C][NB * H];
     __shared__ uint32_t sh_tl0[C][kTlwe0Pad];
     __shared__ uint32_t sh_res[C][N + 4];
+    __shared__ uint32_t sh_red[8 * (n0 + 1)];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int slot = wave >> 1, j = wave & 1;
     const int t = threadIdx.x & 127;
-    {
+    // waves 0 .. 2C-1 rotate; the others keep step through the rotation's
+    // barriers and then join the key switch (eight waves)
+    const bool rotating = wave < 2 * C;
+    if (rotating) {
         const uint32_t* a = MUX ? inc : in0;
         const uint32_t* b = MUX ? (slot == 0 ? in1 : in0) : in1;
         const int32_t sa = MUX ? (slot == 0 ? 1 : -1) : ca;
@@ -304,17 +308,22 @@ __global__ __launch_bounds__(MUX ? 256 : 128, 2) void k_gate_lvl01_fft(
         }
     }
     __syncthreads();
-    AccStore<P> acc;
-    blind_rotate_fft<P, KB>(sh_tl0[slot], mu, acc, j, sh_spec[wave], sh_spec[wave ^ 1], K);
-    sample_extract_store<P>(acc, j, sh_res[slot]);
+    if (rotating) {
+        AccStore<P> acc;
+        blind_rotate_fft<P, KB>(sh_tl0[slot], mu, acc, j, sh_spec[wave], sh_spec[wave ^ 1], K);
+        sample_extract_store<P>(acc, j, sh_res[slot]);
+    }
+    else {
+        rotation_barriers<P, KB>();
+    }
     __syncthreads();
     if constexpr (MUX)
-        key_switch_columns<TFHEpp::lvl10param, SUBSET, 3>(out, sh_res[0], sh_res[1], mux_sign,
-                                                          mux_sign, mux_sign > 0 ? mu : 0u - mu,
-                                                          ksk, threadIdx.x, 256);
+        key_switch_waves<TFHEpp::lvl10param, SUBSET, 8>(out, sh_res[0], sh_res[1], mux_sign,
+                                                        mux_sign, mux_sign > 0 ? mu : 0u - mu, ksk,
+                                                        sh_red, wave, threadIdx.x & 63);
     else
-        key_switch_columns<TFHEpp::lvl10param, SUBSET, 5>(out, sh_res[0], nullptr, 1, 0, 0u, ksk,
-                                                          t, 128);
+        key_switch_waves<TFHEpp::lvl10param, SUBSET, 8>(out, sh_res[0], nullptr, 1, 0, 0u, ksk,
+                                                        sh_red, wave, threadIdx.x & 63);
 }
 
 // SampleExtract + key switch lvl1 -> lvl0 of n TRLWEs (reference SEIandKS,
@@ -550,7 +559,7 @@ __global__ __launch_bounds__(512, 1) void k_gate_lvl01_kbw(
 // :1248-1273, :1305-1331, MUX :1949-2060). MUX runs its two rotations one
 // after the other in the same wave pair (LDS holds one lvl2 job).
 template <bool KB, bool MUX>
-__global__ __launch_bounds__(128) void k_gate_lvl02_fft(
+__global__ __launch_bounds__(256, 1) void k_gate_lvl02_fft(
     uint32_t* out, const uint32_t* in0, const uint32_t* in1, const uint32_t* inc, int32_t ca,
     int32_t cb, uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
     FourierKeys<lvl2param> K)
@@ -564,11 +573,15 @@ __global__ __launch_bounds__(128) void k_gate_lvl02_fft(
     __shared__ T sh_coef[2][N];
     __shared__ uint32_t sh_tl0[kTlwe0Pad];
     __shared__ T sh_res[MUX ? 2 : 1][N + 8];
+    // waves 0 and 1 rotate; waves 2 and 3 keep step through the rotation's
+    // barriers and then join the key switch (four waves: one per SIMD, so
+    // the rotating waves keep the whole register file)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int j = wave & 1;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x & 127;
+    const bool rotating = wave < 2;
     AccStore<P> acc;
-    acc.lds = sh_coef[wave];
+    acc.lds = sh_coef[j];
 #pragma unroll 1
     for (int pass = 0; pass < (MUX ? 2 : 1); pass++) {
         // MUX: pass 0 = c + t - mu, pass 1 = -c + f - mu
@@ -578,24 +591,33 @@ __global__ __launch_bounds__(128) void k_gate_lvl02_fft(
         const int32_t sb = MUX ? 1 : cb;
         const uint32_t off = MUX ? 0u - static_cast<uint32_t>(lvl0param::μ) : offset;
         __syncthreads();
-        for (int i = t; i <= n0; i += 128) {
-            uint32_t v = static_cast<uint32_t>(sa) * a[i] + static_cast<uint32_t>(sb) * b[i];
-            if (i == n0) v += off;
-            sh_tl0[i] = v;
+        if (rotating) {
+            for (int i = t; i <= n0; i += 128) {
+                uint32_t v = static_cast<uint32_t>(sa) * a[i] + static_cast<uint32_t>(sb) * b[i];
+                if (i == n0) v += off;
+                sh_tl0[i] = v;
+            }
         }
         __syncthreads();
-        blind_rotate_fft<P, KB>(sh_tl0, mu, acc, j, sh_spec[wave], sh_spec[wave ^ 1], K);
-        sample_extract_store<P>(acc, j, sh_res[pass]);
+        if (rotating) {
+            blind_rotate_fft<P, KB>(sh_tl0, mu, acc, j, sh_spec[j], sh_spec[j ^ 1], K);
+            sample_extract_store<P>(acc, j, sh_res[pass]);
+        }
+        else {
+            rotation_barriers<P, KB>();
+        }
     }
     __syncthreads();
+    // partial sums of the key switch in the (now free) transform scratch
+    uint32_t* const red = reinterpret_cast<uint32_t*>(&sh_spec[0][0]);
     if constexpr (MUX) {
-        key_switch_columns<TFHEpp::lvl20param, false, 5>(out, sh_res[0], sh_res[1], mux_sign,
-                                                         mux_sign, mux_sign > 0 ? mu : 0 - mu,
-                                                         ksk, t, 128);
+        key_switch_waves<TFHEpp::lvl20param, false, 4>(out, sh_res[0], sh_res[1], mux_sign,
+                                                       mux_sign, mux_sign > 0 ? mu : 0 - mu, ksk,
+                                                       red, wave, threadIdx.x & 63);
     }
     else {
-        key_switch_columns<TFHEpp::lvl20param, false, 5>(out, sh_res[0], nullptr, 1, 0, 0, ksk, t,
-                                                         128);
+        key_switch_waves<TFHEpp::lvl20param, false, 4>(out, sh_res[0], nullptr, 1, 0, 0, ksk, red,
+                                                       wave, threadIdx.x & 63);
     }
 }
 
@@ -777,7 +799,7 @@ void launch_gate_lvl01_fft(bool kb, bool mux, bool subset, uint32_t* out, const 
                            const FourierKeys<lvl1param>& K, hipStream_t st)
 {
 #define CUFHE_G01(KBV, MUXV, SUB)                                                              \
-    hipLaunchKernelGGL((k_gate_lvl01_fft<KBV, MUXV, SUB>), dim3(1), dim3(MUXV ? 256 : 128), 0, \
+    hipLaunchKernelGGL((k_gate_lvl01_fft<KBV, MUXV, SUB>), dim3(1), dim3(512), 0, \
                        st, out, in0, in1, inc, ca, cb, offset, mux_sign, ksk, K)
     if (kb) {
         // key bundle: the one-wave rotation with seven key-streaming waves
@@ -818,7 +840,7 @@ void launch_gate_lvl02_fft(bool kb, bool mux, uint32_t* out, const uint32_t* in0
                            uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
                            const FourierKeys<lvl2param>& K, hipStream_t st)
 {
-    const dim3 grid(1), block(128);
+    const dim3 grid(1), block(256);
 #define CUFHE_L(KBV, MUXV)                                                                     \
     hipLaunchKernelGGL((k_gate_lvl02_fft<KBV, MUXV>), grid, block, 0, st, out, in0, in1, inc, \
                        ca, cb, offset, mux_sign, ksk, K)
