@@ -257,6 +257,23 @@ __device__ __forceinline__ void blind_rotate_ntt(const uint32_t* tl0, uint32_t m
     }
 }
 
+// Workgroup barriers of one blind_rotate_ntt<MODE>: br_step_ntt calls
+// __syncthreads() twice per digit, and blind_rotate_ntt none of its own. A
+// wave of the workgroup that takes no part in the rotation calls
+// rotation_barriers_ntt<MODE>() in its place to stay in step with the
+// rotating waves (every barrier counts all waves that have not ended).
+template <int MODE>
+inline constexpr int kBrNttBarriers =
+    (MODE == NTT_KB ? lvl0param::n / 2 : (MODE == NTT_BB ? lvl0param::n / kBlockEll : lvl0param::n)) *
+    lvl1param::l * 2;
+
+template <int MODE>
+__device__ __forceinline__ void rotation_barriers_ntt()
+{
+#pragma unroll 1
+    for (int i = 0; i < kBrNttBarriers<MODE>; i++) __syncthreads();
+}
+
 // The rotated TRLWE itself: component j's N coefficients at out + j N.
 __device__ __forceinline__ void trlwe_store_ntt(const uint32_t (&acc)[16], int j, uint32_t* out)
 {

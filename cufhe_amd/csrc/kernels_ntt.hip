@@ -180,7 +180,7 @@ __global__ __launch_bounds__(128 * C, 1) void k_br_ntt_shared(const BrJob* __res
 // ---------------------------------------------------------------------------
 // Plain per-gate form, lvl0 API (BR-IKS at lvl1), as k_gate_lvl01_fft.
 template <int MODE, bool SUBSET, bool MUX>
-__global__ __launch_bounds__(MUX ? 256 : 128, 2) void k_gate_lvl01_ntt(
+__global__ __launch_bounds__(512, 1) void k_gate_lvl01_ntt(
     uint32_t* out, const uint32_t* in0, const uint32_t* in1, const uint32_t* inc, int32_t ca,
     int32_t cb, uint32_t offset, int32_t mux_sign, const uint32_t* ksk, NttKeys K)
 {
@@ -192,10 +192,14 @@ __global__ __launch_bounds__(MUX ? 256 : 128, 2) void k_gate_lvl01_ntt(
     __shared__ uint32_t sh_lds[2 * C][kNttLdsWords];
     __shared__ uint32_t sh_tl0[C][kNttTlwe0Pad];
     __shared__ uint32_t sh_res[C][N + 4];
+    __shared__ uint32_t sh_red[8 * (n0 + 1)];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int slot = wave >> 1, j = wave & 1;
     const int t = threadIdx.x & 127;
-    {
+    // waves 0 .. 2C-1 rotate; the others keep step through the rotation's
+    // barriers and then join the key switch (eight waves)
+    const bool rotating = wave < 2 * C;
+    if (rotating) {
         const uint32_t* a = MUX ? inc : in0;
         const uint32_t* b = MUX ? (slot == 0 ? in1 : in0) : in1;
         const int32_t sa = MUX ? (slot == 0 ? 1 : -1) : ca;
@@ -208,17 +212,22 @@ __global__ __launch_bounds__(MUX ? 256 : 128, 2) void k_gate_lvl01_ntt(
         }
     }
     __syncthreads();
-    uint32_t acc[16];
-    blind_rotate_ntt<MODE>(sh_tl0[slot], mu, acc, j, sh_lds[wave], sh_lds[wave ^ 1], K);
-    sample_extract_ntt(acc, j, sh_res[slot]);
+    if (rotating) {
+        uint32_t acc[16];
+        blind_rotate_ntt<MODE>(sh_tl0[slot], mu, acc, j, sh_lds[wave], sh_lds[wave ^ 1], K);
+        sample_extract_ntt(acc, j, sh_res[slot]);
+    }
+    else {
+        rotation_barriers_ntt<MODE>();
+    }
     __syncthreads();
     if constexpr (MUX)
-        key_switch_columns<TFHEpp::lvl10param, SUBSET, 3>(out, sh_res[0], sh_res[1], mux_sign,
-                                                          mux_sign, mux_sign > 0 ? mu : 0u - mu,
-                                                          ksk, threadIdx.x, 256);
+        key_switch_waves<TFHEpp::lvl10param, SUBSET, 8>(out, sh_res[0], sh_res[1], mux_sign,
+                                                        mux_sign, mux_sign > 0 ? mu : 0u - mu, ksk,
+                                                        sh_red, wave, threadIdx.x & 63);
     else
-        key_switch_columns<TFHEpp::lvl10param, SUBSET, 5>(out, sh_res[0], nullptr, 1, 0, 0u, ksk,
-                                                          t, 128);
+        key_switch_waves<TFHEpp::lvl10param, SUBSET, 8>(out, sh_res[0], nullptr, 1, 0, 0u, ksk,
+                                                        sh_red, wave, threadIdx.x & 63);
 }
 
 // ---------------------------------------------------------------------------
@@ -369,7 +378,7 @@ void launch_gate_lvl01_ntt(int mode, bool subset, bool mux, uint32_t* out, const
                            hipStream_t st)
 {
 #define CUFHE_G01N(M, S, X)                                                                     \
-    hipLaunchKernelGGL((k_gate_lvl01_ntt<M, S, X>), dim3(1), dim3(X ? 256 : 128), 0, st, out, in0, \
+    hipLaunchKernelGGL((k_gate_lvl01_ntt<M, S, X>), dim3(1), dim3(512), 0, st, out, in0,          \
                        in1, inc, ca, cb, offset, mux_sign, ksk, K)
 #define CUFHE_G01N_MODE(M)                                                \
     if (subset) {                                                         \

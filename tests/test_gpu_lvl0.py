@@ -30,6 +30,9 @@ def _setups(keys_lvl1, keys_bb):
         "fft_plain": dict(keys=keys_lvl1, bk=keys_lvl1["bk"], ksk=keys_lvl1["ksk"],
                           backend=C.BACKEND_FFT, variant=C.BR_PLAIN, subset=False,
                           mode=O.EXACT, okb=O.BR_PLAIN),
+        "ntt_kb": dict(keys=keys_lvl1, bk=keys_lvl1["bk_kb"], ksk=keys_lvl1["ksk"],
+                       backend=C.BACKEND_NTT, variant=C.BR_KEY_BUNDLE, subset=False,
+                       mode=O.NTT, okb=O.BR_KB),
         "ntt_bb": dict(keys=keys_bb, bk=keys_bb["bk"], ksk=keys_bb["ksk_subset"],
                        backend=C.BACKEND_NTT, variant=C.BR_BLOCK_BINARY, subset=True,
                        mode=O.NTT, okb=O.BR_BB),
@@ -78,7 +81,7 @@ def _want(op, x, y, s):
     return C.TRUTH[op](x, y)
 
 
-@pytest.mark.parametrize("name", ["fft_kb", "fft_plain", "ntt_bb"])
+@pytest.mark.parametrize("name", ["fft_kb", "fft_plain", "ntt_kb", "ntt_bb"])
 def test_lvl0_gates_plain_and_batched(engine, keys_lvl1, keys_bb, name):
     cfg = _setups(keys_lvl1, keys_bb)[name]
     _load(name, cfg)
