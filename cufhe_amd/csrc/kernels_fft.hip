@@ -622,19 +622,25 @@ __global__ __launch_bounds__(256, 1) void k_gate_lvl02_fft(
 }
 
 // ---------------------------------------------------------------------------
-// Batched key switching: one wave per job, lanes over output columns.
+// Batched key switching (gather form): one job per workgroup of eight waves,
+// each wave taking 1/8 of the input coefficients for all output columns,
+// partial sums reduced through LDS. One job's key switch is a long dependent
+// gather loop (1024 x 7 KSK rows at lvl1, 2048 x 7 at lvl2), so a job on a
+// single wave is latency-bound for milliseconds; split eight ways it is not.
 template <class KS, bool SUBSET>
-__global__ __launch_bounds__(256) void k_ks_batch(const KsJob* __restrict__ jobs, int njobs,
+__global__ __launch_bounds__(512) void k_ks_batch(const KsJob* __restrict__ jobs, int njobs,
                                                   const uint32_t* ksk)
 {
     using DT = typename KS::domainP::T;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int job = blockIdx.x * 4 + wave;
+    __shared__ uint32_t sh_red[8 * (KsTraits<KS>::n0 + 1)];
+    const int job = blockIdx.x;
     if (job >= njobs) return;
     const KsJob jb = jobs[job];
-    key_switch_columns<KS, SUBSET, 10>(jb.out, static_cast<const DT*>(jb.in0),
-                                       static_cast<const DT*>(jb.in1), jb.ca, jb.cb,
-                                       static_cast<DT>(jb.offset), ksk, threadIdx.x & 63, 64);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    key_switch_waves<KS, SUBSET, 8>(jb.out, static_cast<const DT*>(jb.in0),
+                                    static_cast<const DT*>(jb.in1), jb.ca, jb.cb,
+                                    static_cast<DT>(jb.offset), ksk, sh_red, wave,
+                                    threadIdx.x & 63);
 }
 
 // Elementwise TLWE combinations (MUX/NMUX combine, NOT, COPY).
@@ -856,7 +862,8 @@ void launch_gate_lvl02_fft(bool kb, bool mux, uint32_t* out, const uint32_t* in0
 void launch_ks_batch(int domain_level, bool subset, const KsJob* jobs, int njobs,
                      const uint32_t* ksk, hipStream_t st)
 {
-    const dim3 grid((njobs + 3) / 4), block(256);
+    if (njobs <= 0) return;
+    const dim3 grid(njobs), block(512);
     if (domain_level == 1) {
         if (subset)
             hipLaunchKernelGGL((k_ks_batch<TFHEpp::lvl10param, true>), grid, block, 0, st, jobs,
