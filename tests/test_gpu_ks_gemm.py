@@ -1,4 +1,4 @@
-"""The batched key switch as an int8 GEMM (batches of >= 256 jobs) is
+"""The batched key switch as an int8 GEMM (every batched plan) is
 bit-identical to the gather form used by the plain per-gate kernels: full
 lvl10 KSK (FFT key bundle), subset KSK (NTT block-binary) and lvl20 (lvl02
 gates), eagerly and as HIP-graph replays interleaved with eager work."""
@@ -11,7 +11,7 @@ from conftest import load_lvl1_keys
 pytestmark = pytest.mark.gpu
 
 W0, W1 = C.TLWE0_WORDS, C.TLWE1_WORDS
-N = 320  # above the GEMM threshold
+N = 320  # several GEMM row tiles
 OPS = ["NAND", "AND", "OR", "XOR", "XNOR", "NOR", "ANDNY", "ORYN", "MUX", "NMUX"]
 
 
