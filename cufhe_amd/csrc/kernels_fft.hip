@@ -330,20 +330,24 @@ __global__ __launch_bounds__(512, 1) void k_gate_lvl01_fft(
 // src/keyswitch_gpu.cu:61-68, include/keyswitch_gpu.cuh:52-123), one job per
 // workgroup.
 template <bool SUBSET>
-__global__ __launch_bounds__(256) void k_se_ks_lvl10(uint32_t* out, const uint32_t* trlwe, int n,
+__global__ __launch_bounds__(512) void k_se_ks_lvl10(uint32_t* out, const uint32_t* trlwe, int n,
                                                      const uint32_t* ksk)
 {
     constexpr int N = lvl1param::n, n0 = lvl0param::n;
     __shared__ uint32_t tl[N + 4];
+    __shared__ uint32_t sh_red[8 * (n0 + 1)];
     const int job = blockIdx.x;
     if (job >= n) return;  // whole workgroup
     const uint32_t* tr = trlwe + static_cast<size_t>(job) * 2 * N;
-    for (int i = threadIdx.x; i <= N; i += 256)
+    for (int i = threadIdx.x; i <= N; i += 512)
         tl[i] = i == 0 ? tr[0] : (i < N ? 0u - tr[N - i] : tr[N]);
     __syncthreads();
-    key_switch_columns<TFHEpp::lvl10param, SUBSET, 3>(out + static_cast<size_t>(job) * (n0 + 1),
-                                                      tl, nullptr, 1, 0, 0u, ksk, threadIdx.x,
-                                                      256);
+    // the input coefficients split over the eight waves (a job's key switch
+    // is a long dependent gather loop; eight shorter ones hide its latency)
+    key_switch_waves<TFHEpp::lvl10param, SUBSET, 8>(out + static_cast<size_t>(job) * (n0 + 1), tl,
+                                                    nullptr, 1, 0, 0u, ksk, sh_red,
+                                                    __builtin_amdgcn_readfirstlane(threadIdx.x >> 6),
+                                                    threadIdx.x & 63);
 }
 
 // ---------------------------------------------------------------------------
@@ -836,9 +840,9 @@ void launch_se_ks_lvl10(bool subset, uint32_t* out, const uint32_t* trlwe, int n
 {
     if (n <= 0) return;
     if (subset)
-        hipLaunchKernelGGL((k_se_ks_lvl10<true>), dim3(n), dim3(256), 0, st, out, trlwe, n, ksk);
+        hipLaunchKernelGGL((k_se_ks_lvl10<true>), dim3(n), dim3(512), 0, st, out, trlwe, n, ksk);
     else
-        hipLaunchKernelGGL((k_se_ks_lvl10<false>), dim3(n), dim3(256), 0, st, out, trlwe, n, ksk);
+        hipLaunchKernelGGL((k_se_ks_lvl10<false>), dim3(n), dim3(512), 0, st, out, trlwe, n, ksk);
 }
 
 void launch_gate_lvl02_fft(bool kb, bool mux, uint32_t* out, const uint32_t* in0,

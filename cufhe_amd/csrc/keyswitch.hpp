@@ -49,75 +49,16 @@ struct KsTraits {
 };
 
 // Key switch of the combination ca*in0 + cb*in1 (+offset on the body) for
-// the output columns i = col0, col0 + stride, ... (< n0 + 1) of one job.
-// All threads of a wave see the same j and digit sequence, so branches are
-// wave-uniform and each KSK row read is a coalesced run over columns.
-template <class KS, bool SUBSET, int MAXCOLS>
-__device__ __forceinline__ void key_switch_columns(uint32_t* out, const typename KS::domainP::T* in0,
-                                                   const typename KS::domainP::T* in1, int32_t ca,
-                                                   int32_t cb, typename KS::domainP::T offset,
-                                                   const uint32_t* ksk, int col0, int stride)
-{
-    using TR = KsTraits<KS>;
-    using DT = typename TR::DT;
-    constexpr uint32_t first = SUBSET ? TR::n0 : 0;
-    const DT ua = static_cast<DT>(static_cast<std::make_signed_t<DT>>(ca));
-    const DT ub = static_cast<DT>(static_cast<std::make_signed_t<DT>>(cb));
-    uint32_t res[MAXCOLS];
-#pragma unroll
-    for (int c = 0; c < MAXCOLS; c++) {
-        const int i = col0 + c * stride;
-        uint32_t r = 0;
-        if (i == static_cast<int>(TR::n0)) {
-            DT v = ua * in0[TR::dn] + offset;
-            if (in1) v += ub * in1[TR::dn];
-            r = TR::convert(v);
-        }
-        else if (SUBSET && i < static_cast<int>(TR::n0)) {
-            DT v = ua * in0[i];
-            if (in1) v += ub * in1[i];
-            r = TR::convert(v);
-        }
-        res[c] = r;
-    }
-#pragma unroll 2
-    for (uint32_t j = first; j < TR::dn; j++) {
-        DT tmp = ua * in0[j] + TR::kOffset;
-        if (in1) tmp += ub * in1[j];
-        const uint32_t* kj = ksk + static_cast<size_t>(j - first) * TR::coef_words;
-#pragma unroll
-        for (uint32_t k = 0; k < KS::t; k++) {
-            const int32_t val =
-                static_cast<int32_t>((tmp >> (TR::W - (k + 1) * KS::basebit)) &
-                                     ((static_cast<DT>(1) << KS::basebit) - 1)) -
-                static_cast<int32_t>(TR::numbase);
-            if (val == 0) continue;
-            const uint32_t* row = kj + k * TR::digit_words + (abs(val) - 1) * TR::row_words;
-#pragma unroll
-            for (int c = 0; c < MAXCOLS; c++) {
-                const int i = col0 + c * stride;
-                if (i <= static_cast<int>(TR::n0)) {
-                    const uint32_t x = row[i];
-                    res[c] = val > 0 ? res[c] - x : res[c] + x;
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < MAXCOLS; c++) {
-        const int i = col0 + c * stride;
-        if (i <= static_cast<int>(TR::n0)) out[i] = res[c];
-    }
-}
-
-// The same key switch spread over NW waves of a workgroup for one job (the
-// plain per-gate form, where a single job's key switch is latency-bound):
-// wave w takes switched coefficients j in its 1/NW share, each lane
-// accumulates columns lane + 64 c (c < 10), the partial sums go through
-// red[NW][n0 + 1] in LDS, and threads [0, 64 NW) sum them into out.
-// Integer sums mod 2^32: bit-identical to key_switch_columns. `wave` is the
-// wave's index within the NW waves of this job; every thread of the
-// workgroup must call it (one __syncthreads inside).
+// one job, spread over NW waves of a workgroup (one job's key switch is a
+// long dependent gather loop over dn x t KSK rows; split NW ways its latency
+// is hidden): wave w takes the switched coefficients j in its 1/NW share,
+// each lane accumulates output columns lane + 64 c (c < 10), the partial
+// sums go through red[NW][n0 + 1] in LDS, and threads [0, 64 NW) sum them
+// into out. All lanes of a wave see the same j and digit sequence, so
+// branches are wave-uniform and each KSK row read is a coalesced run over
+// columns. Integer sums mod 2^32, so the split does not change a bit.
+// `wave` is the wave's index within the NW waves of this job; every thread
+// of the workgroup must call it (one __syncthreads inside).
 template <class KS, bool SUBSET, int NW>
 __device__ __forceinline__ void key_switch_waves(uint32_t* out, const typename KS::domainP::T* in0,
                                                  const typename KS::domainP::T* in1, int32_t ca,
