@@ -84,14 +84,21 @@ __device__ __forceinline__ uint32_t mod_sub(uint32_t a, uint32_t b)
     const uint32_t d = a + ntt31::P - b;
     return d >= ntt31::P ? d - ntt31::P : d;
 }
+// r mod P for r < 2P: min(r, r - P) (r - P wraps above r when r < P). Written
+// as an unsigned min so it stays two VALU ops; a compare on the 64-bit
+// Montgomery sum (what `r >= P ? r - P : r` folds into) costs a 64-bit
+// compare, a select and a VCC hazard wait.
+__device__ __forceinline__ uint32_t reduce_2p(uint32_t r)
+{
+    return __builtin_elementwise_min(r, r - ntt31::P);
+}
 // a * b * R^-1 mod P, canonical (a, b < P)
 __device__ __forceinline__ uint32_t mont_mul(uint32_t a, uint32_t b)
 {
     const uint64_t t = static_cast<uint64_t>(a) * b;
     const uint32_t m = static_cast<uint32_t>(t) * ntt31::PINV;
     const uint64_t u = t + static_cast<uint64_t>(m) * ntt31::P;  // < 2^63 + 2^63
-    const uint32_t r = static_cast<uint32_t>(u >> 32);           // < 2P
-    return r >= ntt31::P ? r - ntt31::P : r;
+    return reduce_2p(static_cast<uint32_t>(u >> 32));            // u >> 32 < 2P
 }
 
 // (a1 b1 + a2 b2) R^-1 mod P with one Montgomery reduction: the 64-bit sum
@@ -103,8 +110,7 @@ __device__ __forceinline__ uint32_t mont_dot2(uint32_t a1, uint32_t b1, uint32_t
     const uint64_t t = static_cast<uint64_t>(a1) * b1 + static_cast<uint64_t>(a2) * b2;
     const uint32_t m = static_cast<uint32_t>(t) * ntt31::PINV;
     const uint64_t u = t + static_cast<uint64_t>(m) * ntt31::P;
-    const uint32_t r = static_cast<uint32_t>(u >> 32);
-    return r >= ntt31::P ? r - ntt31::P : r;
+    return reduce_2p(static_cast<uint32_t>(u >> 32));
 }
 
 // Multiply by the compile-time twiddle W_M^E (forward: omega_M^E, inverse:
