@@ -271,9 +271,13 @@ __device__ __forceinline__ uint32_t torus_to_ntt31(uint32_t a)
     const uint64_t v = (static_cast<uint64_t>(a) * ntt31::P + (1ULL << 31)) >> 32;
     return v >= ntt31::P ? static_cast<uint32_t>(v - ntt31::P) : static_cast<uint32_t>(v);
 }
+// v mod P for |v| < P: as u32 a negative v is above 2^31 > P and v + P
+// wraps to P - |v|, so the unsigned min picks the residue either way
+// (two VALU ops, no compare and select).
 __device__ __forceinline__ uint32_t signed_to_ntt31(int32_t v)
 {
-    return v < 0 ? ntt31::P - static_cast<uint32_t>(-v) : static_cast<uint32_t>(v);
+    const uint32_t u = static_cast<uint32_t>(v);
+    return __builtin_elementwise_min(u, u + ntt31::P);
 }
 __device__ __forceinline__ uint32_t ntt31_to_torus32(uint32_t val)
 {
