@@ -60,6 +60,9 @@ def lib():
                               ctypes.c_int),
         "oracle_gate_batch_lvl1": ([ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp, vp,
                                     ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "oracle_gate_batch_lvl1_ex": ([ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp, vp, vp,
+                                       ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "oracle_set_mutation": ([ctypes.c_int], ctypes.c_int),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -179,11 +182,34 @@ def gate_lvl02(mode, kb, op, in0, in1, inc, bk, ksk):
     return out
 
 
-def gate_batch_lvl1(mode, kb, ops, in0, in1, bk, ksk, nthreads):
+def gate_batch_lvl1(mode, kb, ops, in0, in1, bk, ksk, nthreads, inc=None):
+    """Rows of lvl1 gates on nthreads host threads (inc: MUX selectors)."""
     ops = np.ascontiguousarray(ops, np.int32)
     in0 = np.ascontiguousarray(in0, np.uint32)
     in1 = np.ascontiguousarray(in1, np.uint32)
+    inc = None if inc is None else np.ascontiguousarray(inc, np.uint32)
     out = np.zeros_like(in0)
-    _ok(lib().oracle_gate_batch_lvl1(mode, int(kb), _p(ops), _p(in0), _p(in1), _p(bk), _p(ksk),
-                                     _p(out), len(ops), nthreads), "gate_batch_lvl1")
+    _ok(lib().oracle_gate_batch_lvl1_ex(mode, int(kb), _p(ops), _p(in0), _p(in1), _p(inc), _p(bk),
+                                        _p(ksk), _p(out), len(ops), nthreads), "gate_batch_lvl1")
     return out
+
+
+MUTATIONS = {
+    "ntt_torus_offset": 1, "bar_off_by_one": 2, "b_rounded": 3, "bb_sign": 4,
+    "bb_not_halved": 5, "ks_no_round": 6, "kb_order": 7, "a_truncated": 8,
+}
+
+
+class mutation:
+    """Context manager planting one known mistake in the oracle (tests only)."""
+
+    def __init__(self, name):
+        self.code = MUTATIONS[name]
+
+    def __enter__(self):
+        _ok(lib().oracle_set_mutation(self.code), "set_mutation")
+        return self
+
+    def __exit__(self, *exc):
+        _ok(lib().oracle_set_mutation(0), "set_mutation")
+        return False

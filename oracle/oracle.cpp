@@ -31,6 +31,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <thread>
 #include <type_traits>
 #include <vector>
@@ -45,6 +46,21 @@ using u128 = unsigned __int128;
 using cd = std::complex<double>;
 
 enum Mode { ORACLE_EXACT = 0, ORACLE_FFT = 1, ORACLE_NTT = 2 };
+
+// Planted mistakes, for the tests that show the pinning checks catch them
+// (oracle_set_mutation). 0 = the faithful restatement.
+enum Mutation {
+    MUT_NONE = 0,
+    MUT_NTT_TORUS_OFFSET = 1,  // ntt_mod_to_torus32 rounds with 2^31 instead of 2^30
+    MUT_BAR_OFF_BY_ONE = 2,    // bar = 2N - ms(b) + 1
+    MUT_B_ROUNDED = 3,         // a rounding offset added to b before the mod switch
+    MUT_BB_SIGN = 4,           // block-binary body + correction/2 instead of -
+    MUT_BB_NOT_HALVED = 5,     // block-binary body - correction
+    MUT_KS_NO_ROUND = 6,       // key-switch rounding offset dropped
+    MUT_KB_ORDER = 7,          // key-bundle members 3i+1 and 3i+2 swapped
+    MUT_A_TRUNCATED = 8,       // mask coefficients mod-switched without rounding
+};
+int g_mut = MUT_NONE;
 
 // ---------------------------------------------------------------- parameters
 // Same values as include/tfhe_params.hpp (checked against the library by
@@ -67,9 +83,19 @@ struct L2 {
 // ---------------------------------------------------------------- modular NTT
 struct Field {
     u64 p;
-    u64 mul(u64 a, u64 b) const { return static_cast<u64>((u128)a * b % p); }
+    u64 mul(u64 a, u64 b) const
+    {
+        // a, b < P31 < 2^32: the product fits, and a constant divisor lets
+        // the compiler reduce with a multiplication
+        if (p == 2147473409ULL) return a * b % 2147473409ULL;
+        return static_cast<u64>((u128)a * b % p);
+    }
     u64 add(u64 a, u64 b) const
     {
+        if (p < (1ULL << 63)) {
+            const u64 s = a + b;
+            return s >= p ? s - p : s;
+        }
         u128 s = (u128)a + b;
         return static_cast<u64>(s >= p ? s - p : s);
     }
@@ -191,7 +217,8 @@ u32 ntt_mod_to_torus32(u64 val, u64 P)
         const u32 mag = static_cast<u32>(neg ? (kP31 - val) : val);
         const u64 inv_mul = (1ULL << 63) / kP31;
         const u64 temp = static_cast<u64>(mag) * inv_mul;
-        const u32 t32 = static_cast<u32>((temp + (1ULL << 30)) >> 31);
+        const u64 half_unit = g_mut == MUT_NTT_TORUS_OFFSET ? (1ULL << 31) : (1ULL << 30);
+        const u32 t32 = static_cast<u32>((temp + half_unit) >> 31);
         return neg ? static_cast<u32>(0u - t32) : t32;
     }
     const bool neg = val > kGold / 2;
@@ -477,6 +504,91 @@ std::vector<std::vector<u64>> key_mod(const typename P::T* C)
     return out;
 }
 
+// Spectral form of the NTT backend for whole rotations: the key polynomials
+// are mapped to Z_P (torus_to_ntt_mod) and transformed once per call, and the
+// X^a - 1 factors are multiplied in the transform domain. Arithmetic mod P is
+// exact and NTT(x) . NTT(y) = NTT(x * y) in Z_P[X]/(X^N + 1), so the results
+// are the same residues as the coefficient-domain combination of the
+// reference (include/gatebootstrapping_gpu.cuh:747-761, :1187-1332).
+template <class P>
+std::vector<u64> key_spectra_ntt(const typename P::T* bk, size_t ntrgsw)
+{
+    using T = typename P::T;
+    constexpr u32 N = P::N, polys = 4 * P::l;
+    const u64 Pm = ntt_prime<P>();
+    const NegaNTT& nt = nega(Pm, N);
+    std::vector<u64> out(ntrgsw * polys * N);
+    std::vector<u64> c(N);
+    for (size_t t = 0; t < ntrgsw * polys; t++) {
+        const T* src = bk + t * N;
+        for (u32 i = 0; i < N; i++) c[i] = torus_to_ntt_mod<T>(src[i], Pm) % Pm;
+        nt.forward(c);
+        std::copy(c.begin(), c.end(), out.begin() + t * N);
+    }
+    return out;
+}
+
+// Spectrum of X^a - 1 (a in [0, 2N)) for the NTT prime of ring size N.
+const u64* xai_ntt(u32 N, u32 a)
+{
+    struct Tab {
+        std::once_flag once;
+        std::vector<u64> v;
+    };
+    static Tab t1, t2;
+    Tab& t = N == 1024 ? t1 : t2;
+    const u64 Pm = N == 1024 ? kP31 : kGold;
+    std::call_once(t.once, [&]() {
+        const NegaNTT& nt = nega(Pm, N);
+        t.v.resize(static_cast<size_t>(2 * N) * N);
+        std::vector<u64> c(N);
+        for (u32 e = 0; e < 2 * N; e++) {
+            std::fill(c.begin(), c.end(), 0);
+            c[0] = Pm - 1;
+            if (e < N)
+                c[e] = (c[e] + 1) % Pm;
+            else
+                c[e - N] = (c[e - N] + Pm - 1) % Pm;
+            nt.forward(c);
+            std::copy(c.begin(), c.end(), t.v.begin() + static_cast<size_t>(e) * N);
+        }
+    });
+    return t.v.data() + static_cast<size_t>(a) * N;
+}
+
+// out (+)= ExtProd(d, K) with K given as spectra (rows * 2 polys of N bins).
+template <class P>
+void extprod_ntt_spec(const typename P::T* d, const u64* K, typename P::T* out, bool add)
+{
+    using T = typename P::T;
+    constexpr u32 N = P::N, rows = 2 * P::l;
+    const u64 Pm = ntt_prime<P>();
+    const NegaNTT& nt = nega(Pm, N);
+    const Field& f = nt.f;
+    std::vector<u64> acc[2] = {std::vector<u64>(N, 0), std::vector<u64>(N, 0)};
+    std::vector<u64> D(N);
+    for (u32 r = 0; r < rows; r++) {
+        const u32 j = r / P::l, q = r % P::l;
+        for (u32 i = 0; i < N; i++) D[i] = signed_int_to_ntt_mod(gadget_digit<P>(d[j * N + i], q), Pm);
+        nt.forward(D);
+        for (u32 o = 0; o < 2; o++) {
+            const u64* k = K + static_cast<size_t>(r * 2 + o) * N;
+            for (u32 i = 0; i < N; i++) acc[o][i] = f.add(acc[o][i], f.mul(k[i], D[i]));
+        }
+    }
+    for (u32 o = 0; o < 2; o++) {
+        nt.inverse(acc[o]);
+        for (u32 i = 0; i < N; i++) {
+            T v;
+            if constexpr (sizeof(T) == 4)
+                v = ntt_mod_to_torus32(acc[o][i], Pm);
+            else
+                v = ntt_mod_to_torus64(acc[o][i], Pm);
+            out[o * N + i] = add ? static_cast<T>(out[o * N + i] + v) : v;
+        }
+    }
+}
+
 // ---------------------------------------------------------------- blind rotation
 u32 mod_switch(u32 phase, u32 nbit) { return phase >> (32 - 1 - nbit); }
 
@@ -495,21 +607,35 @@ u32 bb_corrected_body(const u32* tl0)
     u32 sum = 0;
     for (u32 i = 0; i < N0; i++) sum += tl0[i] - ((mod_switch(tl0[i] + rnd, P::nbit)) << shift);
     const i32 correction = static_cast<i32>(sum);
+    if (g_mut == MUT_BB_SIGN) return tl0[N0] + static_cast<u32>(correction / 2) + rnd;
+    if (g_mut == MUT_BB_NOT_HALVED) return tl0[N0] - static_cast<u32>(correction) + rnd;
     return tl0[N0] - static_cast<u32>(correction / 2) + rnd;
 }
 
+// Number of TRGSWs of a lvl0 -> P bootstrapping key of the given variant.
+inline size_t bk_trgsws(int variant) { return variant == BR_KB ? 3 * (N0 / 2) : N0; }
+
+// kspec: NTT mode only, key_spectra_ntt of bk (computed here when null).
 template <class P>
 void blind_rotate(int mode, int variant, const u32* tl0, const typename P::T* bk, typename P::T mu,
-                  typename P::T* acc)
+                  typename P::T* acc, const std::vector<u64>* kspec = nullptr)
 {
     using T = typename P::T;
-    constexpr u32 N = P::N, rows = 2 * P::l;
+    constexpr u32 N = P::N, rows = 2 * P::l, polys = 2 * rows;
     constexpr size_t TW = static_cast<size_t>(rows) * 2 * N;
-    const u32 rnd = 1u << (32 - 2 - P::nbit);
+    const u32 rnd = g_mut == MUT_A_TRUNCATED ? 0u : 1u << (32 - 2 - P::nbit);
     const bool kb = variant == BR_KB;
+    std::vector<u64> kspec_local;
+    if (mode == ORACLE_NTT && !kspec) {
+        kspec_local = key_spectra_ntt<P>(bk, bk_trgsws(variant));
+        kspec = &kspec_local;
+    }
+    auto kpoly = [&](size_t t) { return kspec->data() + t * polys * N; };
     // test vector (gatebootstrapping_gpu.cuh:31-53)
-    const u32 body = variant == BR_BB ? bb_corrected_body<P>(tl0) : tl0[N0];
-    const u32 bar = 2 * N - mod_switch(body, P::nbit);
+    u32 body = variant == BR_BB ? bb_corrected_body<P>(tl0) : tl0[N0];
+    if (g_mut == MUT_B_ROUNDED && variant != BR_BB) body += 1u << (32 - 2 - P::nbit);
+    u32 bar = 2 * N - mod_switch(body, P::nbit);
+    if (g_mut == MUT_BAR_OFF_BY_ONE) bar = (bar + 1) % (2 * N);
     for (u32 i = 0; i < N; i++) {
         acc[i] = 0;
         if (bar == 2 * N)
@@ -518,14 +644,17 @@ void blind_rotate(int mode, int variant, const u32* tl0, const typename P::T* bk
             acc[N + i] = ((i < (bar & (N - 1))) ^ ((bar >> P::nbit) != 0)) ? static_cast<T>(0 - mu) : mu;
     }
     std::vector<T> tmp(2 * N), rot(2 * N), comb(TW);
+    std::vector<u64> cm(kspec ? static_cast<size_t>(polys) * N : 0);
     if (kb) {
         for (u32 it = 0; it < N0 / 2; it++) {
             const u32 a0 = mod_switch(tl0[2 * it] + rnd, P::nbit);
             const u32 a1 = mod_switch(tl0[2 * it + 1] + rnd, P::nbit);
             const u32 a01 = (a0 + a1) & (2 * N - 1);
-            const T* b0 = bk + (3 * it + 0) * TW;
-            const T* b1 = bk + (3 * it + 1) * TW;
-            const T* b2 = bk + (3 * it + 2) * TW;
+            size_t m0 = 3 * it + 0, m1 = 3 * it + 1, m2 = 3 * it + 2;
+            if (g_mut == MUT_KB_ORDER) std::swap(m1, m2);
+            const T* b0 = bk + m0 * TW;
+            const T* b1 = bk + m1 * TW;
+            const T* b2 = bk + m2 * TW;
             if (mode == ORACLE_FFT) {
                 // combined = one + b2*xai[a1] + b1*xai[a0] + b0*xai[a01] in FP64
                 const auto s0 = key_spectra<P>(b0), s1 = key_spectra<P>(b1), s2 = key_spectra<P>(b2);
@@ -552,27 +681,23 @@ void blind_rotate(int mode, int variant, const u32* tl0, const typename P::T* bk
                 // one + b2 (X^a1-1) + b1 (X^a0-1) + b0 (X^a01-1) in Z_P
                 const u64 Pm = ntt_prime<P>();
                 const Field f{Pm};
-                const auto m0 = key_mod<P>(b0), m1 = key_mod<P>(b1), m2 = key_mod<P>(b2);
-                std::vector<std::vector<u64>> cm(rows * 2, std::vector<u64>(N, 0));
-                auto add_rot = [&](std::vector<u64>& dst, const std::vector<u64>& src, u32 a) {
-                    // dst += src * (X^a - 1) mod P
+                const u64 *k0 = kpoly(m0), *k1 = kpoly(m1), *k2 = kpoly(m2);
+                const u64 *x0 = xai_ntt(N, a0), *x1 = xai_ntt(N, a1), *x01 = xai_ntt(N, a01);
+                for (u32 p = 0; p < polys; p++) {
+                    const u32 r = p / 2, o = p % 2;
+                    // the identity term is a constant polynomial: the same value in every bin
+                    const u64 one =
+                        o == r / P::l ? torus_to_ntt_mod<T>(gadget_h<P>(r % P::l), Pm) % Pm : 0;
+                    const size_t off = static_cast<size_t>(p) * N;
                     for (u32 i = 0; i < N; i++) {
-                        const u32 s = (i - a) & (N - 1);
-                        const bool neg = (i < (a & (N - 1))) ^ ((a / N) != 0);
-                        const u64 v = src[s] % Pm;
-                        dst[i] = neg ? f.sub(dst[i], v) : f.add(dst[i], v);
-                        dst[i] = f.sub(dst[i], src[i] % Pm);
+                        u64 v = one;
+                        v = f.add(v, f.mul(k2[off + i], x1[i]));
+                        v = f.add(v, f.mul(k1[off + i], x0[i]));
+                        v = f.add(v, f.mul(k0[off + i], x01[i]));
+                        cm[off + i] = v;
                     }
-                };
-                for (u32 r = 0; r < rows; r++)
-                    for (u32 o = 0; o < 2; o++) {
-                        auto& c = cm[r * 2 + o];
-                        if (o == r / P::l) c[0] = torus_to_ntt_mod<T>(gadget_h<P>(r % P::l), Pm) % Pm;
-                        add_rot(c, m2[r * 2 + o], a1);
-                        add_rot(c, m1[r * 2 + o], a0);
-                        add_rot(c, m0[r * 2 + o], a01);
-                    }
-                extprod_ntt<P>(acc, cm, Pm, acc, false);
+                }
+                extprod_ntt_spec<P>(acc, cm.data(), acc, false);
                 continue;
             }
             // exact: combined key in the polynomial domain mod 2^w
@@ -598,23 +723,18 @@ void blind_rotate(int mode, int variant, const u32* tl0, const typename P::T* bk
             u32 a[BB_ELL];
             for (u32 e = 0; e < BB_ELL; e++) a[e] = mod_switch(tl0[blk * BB_ELL + e] + rnd, P::nbit);
             if (mode == ORACLE_NTT) {
-                const u64 Pm = ntt_prime<P>();
-                const Field f{Pm};
-                std::vector<std::vector<u64>> cm(rows * 2, std::vector<u64>(N, 0));
+                const Field f{ntt_prime<P>()};
+                std::fill(cm.begin(), cm.end(), 0);
                 for (u32 e = 0; e < BB_ELL; e++) {
-                    const auto m = key_mod<P>(bk + (blk * BB_ELL + e) * TW);
-                    for (u32 p = 0; p < rows * 2; p++)
-                        for (u32 i = 0; i < N; i++) {
-                            const u32 sidx = (i - a[e]) & (N - 1);
-                            const bool neg = (i < (a[e] & (N - 1))) ^ ((a[e] / N) != 0);
-                            const u64 v = m[p][sidx] % Pm;
-                            u64& d = cm[p][i];
-                            d = neg ? f.sub(d, v) : f.add(d, v);
-                            d = f.sub(d, m[p][i] % Pm);
-                        }
+                    const u64* k = kpoly(blk * BB_ELL + e);
+                    const u64* x = xai_ntt(N, a[e]);
+                    for (u32 p = 0; p < polys; p++) {
+                        const size_t off = static_cast<size_t>(p) * N;
+                        for (u32 i = 0; i < N; i++) cm[off + i] = f.add(cm[off + i], f.mul(k[off + i], x[i]));
+                    }
                 }
                 for (u32 i = 0; i < 2 * N; i++) tmp[i] = 0;
-                extprod_ntt<P>(acc, cm, Pm, tmp.data(), true);
+                extprod_ntt_spec<P>(acc, cm.data(), tmp.data(), true);
             }
             else {
                 for (size_t i = 0; i < TW; i++) comb[i] = 0;
@@ -643,7 +763,7 @@ void blind_rotate(int mode, int variant, const u32* tl0, const typename P::T* bk
             extprod_fft<P>(rot.data(), key_spectra<P>(C), tmp.data());
         else if (mode == ORACLE_NTT) {
             for (u32 i = 0; i < 2 * N; i++) tmp[i] = 0;
-            extprod_ntt<P>(rot.data(), key_mod<P>(C), ntt_prime<P>(), tmp.data(), true);
+            extprod_ntt_spec<P>(rot.data(), kpoly(it), tmp.data(), true);
         }
         else
             extprod_exact<P>(rot.data(), C, tmp.data());
@@ -669,7 +789,7 @@ void identity_key_switch(u32 dn, bool subset, const DT* in0, const DT* in1, i32 
     constexpr u32 n0 = N0;
     DT decomp = 0;
     for (u32 i = 1; i <= KS_T; i++) decomp += static_cast<DT>(numbase) << (W - i * KS_BASEBIT);
-    const DT round = static_cast<DT>(1) << (W - (1 + KS_BASEBIT * KS_T));
+    const DT round = g_mut == MUT_KS_NO_ROUND ? 0 : static_cast<DT>(1) << (W - (1 + KS_BASEBIT * KS_T));
     auto conv = [](DT v) -> u32 {
         if constexpr (W == 32)
             return static_cast<u32>(v);
@@ -922,8 +1042,11 @@ int oracle_identity_key_switch(int level, int subset, const void* in0, const voi
 
 // lvl1 gate API (IKS-BR order): lvl1 TLWEs in and out. kb: blind-rotation
 // variant (0 plain, 1 key bundle, 2 block-binary) plus 4 for a subset KSK.
-int oracle_gate_lvl1(int mode, int kb, int op, const u32* in0, const u32* in1, const u32* inc,
-                     const u32* bk, const u32* ksk, u32* out)
+}  // extern "C"
+
+namespace {
+int gate_lvl1_impl(int mode, int kb, int op, const u32* in0, const u32* in1, const u32* inc,
+                   const u32* bk, const u32* ksk, u32* out, const std::vector<u64>* kspec)
 {
     if ((kb & 3) == BR_BB && mode == ORACLE_FFT) return -1;
     const bool subset = (kb & 4) != 0;
@@ -935,10 +1058,10 @@ int oracle_gate_lvl1(int mode, int kb, int op, const u32* in0, const u32* in1, c
     }
     if (op == 10 || op == 11) {  // MUX / NMUX
         identity_key_switch<u32>(N, subset, inc, in1, 1, 1, 0u - mu, ksk, tl0.data());
-        blind_rotate<L1>(mode, kb & 3, tl0.data(), bk, mu, acc.data());
+        blind_rotate<L1>(mode, kb & 3, tl0.data(), bk, mu, acc.data(), kspec);
         sample_extract<u32>(acc.data(), N, r1.data());
         identity_key_switch<u32>(N, subset, inc, in0, -1, 1, 0u - mu, ksk, tl0.data());
-        blind_rotate<L1>(mode, kb & 3, tl0.data(), bk, mu, acc.data());
+        blind_rotate<L1>(mode, kb & 3, tl0.data(), bk, mu, acc.data(), kspec);
         sample_extract<u32>(acc.data(), N, r0.data());
         for (u32 i = 0; i <= N; i++) {
             u32 v = r1[i] + r0[i];
@@ -951,9 +1074,19 @@ int oracle_gate_lvl1(int mode, int kb, int op, const u32* in0, const u32* in1, c
     if (!gate_lin(op, g)) return -1;
     identity_key_switch<u32>(N, subset, in0, in1, g.ca, g.cb, static_cast<u32>(g.off) * mu, ksk,
                              tl0.data());
-    blind_rotate<L1>(mode, kb & 3, tl0.data(), bk, mu, acc.data());
+    blind_rotate<L1>(mode, kb & 3, tl0.data(), bk, mu, acc.data(), kspec);
     sample_extract<u32>(acc.data(), N, out);
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_gate_lvl1(int mode, int kb, int op, const u32* in0, const u32* in1, const u32* inc,
+                     const u32* bk, const u32* ksk, u32* out)
+{
+    return gate_lvl1_impl(mode, kb, op, in0, in1, inc, bk, ksk, out, nullptr);
 }
 
 // lvl02 gate API (BR-IKS order): lvl0 TLWEs in and out, BR at lvl2.
@@ -998,24 +1131,47 @@ int oracle_gate_lvl02(int mode, int kb, int op, const u32* in0, const u32* in1, 
 }
 
 // Batch of lvl1 gates on nthreads host threads (one gate per thread):
-// the CPU baseline of the benchmark. Ciphertexts are rows of 1025 words.
-int oracle_gate_batch_lvl1(int mode, int kb, const int32_t* ops, const u32* in0, const u32* in1,
-                           const u32* bk, const u32* ksk, u32* out, int n, int nthreads)
+// the CPU baseline of the benchmark and the checker of full-size batches.
+// Ciphertexts are rows of 1025 words; inc may be null when no gate is a MUX.
+// The NTT-mode key spectra are computed once for the whole batch.
+int oracle_gate_batch_lvl1_ex(int mode, int kb, const int32_t* ops, const u32* in0, const u32* in1,
+                              const u32* inc, const u32* bk, const u32* ksk, u32* out, int n,
+                              int nthreads)
 {
     if (nthreads < 1) nthreads = 1;
+    if ((kb & 3) == BR_BB && mode == ORACLE_FFT) return -1;
+    std::vector<u64> kspec;
+    if (mode == ORACLE_NTT) kspec = key_spectra_ntt<L1>(bk, bk_trgsws(kb & 3));
+    const std::vector<u64>* ks = mode == ORACLE_NTT ? &kspec : nullptr;
     std::vector<std::thread> pool;
     std::vector<int> rc(nthreads, 0);
     for (int t = 0; t < nthreads; t++)
         pool.emplace_back([&, t]() {
             for (int g = t; g < n; g += nthreads) {
                 const size_t o = static_cast<size_t>(g) * (L1::N + 1);
-                if (oracle_gate_lvl1(mode, kb, ops[g], in0 + o, in1 + o, nullptr, bk, ksk, out + o))
+                if (gate_lvl1_impl(mode, kb, ops[g], in0 + o, in1 + o, inc ? inc + o : nullptr, bk,
+                                   ksk, out + o, ks))
                     rc[t] = -1;
             }
         });
     for (auto& th : pool) th.join();
     for (int r : rc)
         if (r) return r;
+    return 0;
+}
+
+int oracle_gate_batch_lvl1(int mode, int kb, const int32_t* ops, const u32* in0, const u32* in1,
+                           const u32* bk, const u32* ksk, u32* out, int n, int nthreads)
+{
+    return oracle_gate_batch_lvl1_ex(mode, kb, ops, in0, in1, nullptr, bk, ksk, out, n, nthreads);
+}
+
+// Test hook: plant one of the Mutation mistakes (0 restores the faithful
+// oracle). Used only by the tests that show the pinning checks catch them.
+int oracle_set_mutation(int m)
+{
+    if (m < 0 || m > MUT_A_TRUNCATED) return -1;
+    g_mut = m;
     return 0;
 }
 
