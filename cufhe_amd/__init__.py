@@ -115,6 +115,8 @@ def lib():
         "cufhe_plan_bootstraps": ([vp], i32),
         "cufhe_gate_batch_lvl1": ([vp, i32, vp, i32, vp], i32),
         "cufhe_identity_key_switch_lvl10": ([vp, vp, i32, vp], i32),
+        "cufhe_identity_key_switch_lvl20": ([vp, vp, i32, vp], i32),
+        "cufhe_bootstrap": ([vp, vp, u32, vp], i32),
         "cufhe_bootstrap_lvl01": ([vp, vp, u32, i32, i32, vp], i32),
         "cufhe_bootstrap_lvl02": ([vp, vp, u64, i32, vp], i32),
         "cufhe_ffnt_forward": ([i32, vp, i32, vp, i32, vp], i32),

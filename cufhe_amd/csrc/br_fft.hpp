@@ -161,7 +161,11 @@ struct BrShape {
 // each wave multiplies the spectra of BOTH components (its own and
 // partner_spec) by the key column of its own output component only.
 // All 2C waves of the workgroup must call this the same number of times.
-template <class P, bool KB>
+// GHOST: the same loop nest and workgroup barriers with no loads, stores or
+// arithmetic, for waves of the workgroup that take no part in a rotation
+// but must keep step with it (every barrier counts all waves that have not
+// ended); none of the pointer arguments is dereferenced then.
+template <class P, bool KB, bool GHOST = false>
 __device__ __forceinline__ void br_step_fft(AccStore<P>& acc, int j, uint32_t a0, uint32_t a1,
                                             const cplx* trgsw0, cplx* my_spec,
                                             const cplx* partner_spec, const FourierKeys<P>& K)
@@ -174,7 +178,7 @@ __device__ __forceinline__ void br_step_fft(AccStore<P>& acc, int j, uint32_t a0
 
     // Digit source: acc itself (KB) or (X^a0 - 1) acc (plain CMUX).
     T src[(KB || AccStore<P>::kLds) ? 1 : 2 * R];
-    if constexpr (!KB && !AccStore<P>::kLds) {
+    if constexpr (!GHOST && !KB && !AccStore<P>::kLds) {
         T* rot = reinterpret_cast<T*>(my_spec);
 #pragma unroll
         for (int r = 0; r < 2 * R; r++) rot[AccStore<P>::coef_index(r, lane)] = acc.reg[r];
@@ -201,13 +205,13 @@ __device__ __forceinline__ void br_step_fft(AccStore<P>& acc, int j, uint32_t a0
 
     const FFTTables tb = K.fft;
     XaiGen<P> x0g, x1g, x01g;
-    if constexpr (KB) {
+    if constexpr (KB && !GHOST) {
         x0g.init(a0, K.e2n, lane);
         x1g.init(a1, K.e2n, lane);
         x01g.init((a0 + a1) & (2 * N - 1), K.e2n, lane);
     }
     // key column of output component j: [trgsw][row][j][H]
-    const cplx* col0 = trgsw0 + static_cast<size_t>(j) * H + lane;
+    const cplx* col0 = GHOST ? nullptr : trgsw0 + static_cast<size_t>(j) * H + lane;
 
     cplx accj[R];
 #pragma unroll
@@ -215,7 +219,7 @@ __device__ __forceinline__ void br_step_fft(AccStore<P>& acc, int j, uint32_t a0
 
 #pragma unroll 1
     for (int ps = 0; ps < S::kPasses; ps++) {
-        {
+        if constexpr (!GHOST) {
             const int qd = ps * NB;
             cplx z[R];
 #pragma unroll
@@ -250,17 +254,19 @@ __device__ __forceinline__ void br_step_fft(AccStore<P>& acc, int j, uint32_t a0
             return v;
         };
         KeyVals kv[PD];
+        if constexpr (!GHOST) {
 #pragma unroll
-        for (int d = 0; d < PD; d++) kv[d] = load_keys(d);
+            for (int d = 0; d < PD; d++) kv[d] = load_keys(d);
+        }
         __syncthreads();
-        if constexpr (KB) {
+        if constexpr (KB && !GHOST) {
             x0g.pin();
             x1g.pin();
             x01g.pin();
         }
         const double hq = ldexp(1.0, -static_cast<int>((q + 1) * P::Bgbit));
 #pragma unroll
-        for (int d = 0; d < R; d++) {
+        for (int d = 0; d < (GHOST ? 0 : R); d++) {
             const KeyVals cur = kv[d % PD];
             if (d + PD < R) kv[d % PD] = load_keys(d + PD);
             cplx x0, x1, x01;
@@ -291,6 +297,7 @@ __device__ __forceinline__ void br_step_fft(AccStore<P>& acc, int j, uint32_t a0
         __syncthreads();
     }
 
+    if constexpr (GHOST) return;
     fft_inverse<H>(accj, my_spec, tb);
     constexpr double scale = kTorusScale<T> / H;
 #pragma unroll
@@ -312,7 +319,10 @@ __device__ __forceinline__ void br_step_fft(AccStore<P>& acc, int j, uint32_t a0
 
 // Full blind rotation + sample extraction for the wave pair of one job.
 // tl0: the (pre-added) lvl0 TLWE in LDS. out: target TLWE (null: no store).
-template <class P, bool KB>
+// GHOST: the rotation's loop nest and workgroup barriers only (see
+// br_step_fft), for the waves of a workgroup that do not rotate; call it as
+// blind_rotate_ghost<P, KB>().
+template <class P, bool KB, bool GHOST = false>
 __device__ __forceinline__ void blind_rotate_fft(const uint32_t* tl0, typename P::T mu,
                                                  AccStore<P>& acc, int j, cplx* my_spec,
                                                  const cplx* partner_spec,
@@ -322,48 +332,48 @@ __device__ __forceinline__ void blind_rotate_fft(const uint32_t* tl0, typename P
     constexpr int N = P::n, R = N / 128;
     constexpr int n0 = lvl0param::n;
     const int lane = threadIdx.x & 63;
-    const uint32_t bar = __builtin_amdgcn_readfirstlane(2 * N - mod_switch<P>(tl0[n0]));
+    if constexpr (!GHOST) {
+        const uint32_t bar = __builtin_amdgcn_readfirstlane(2 * N - mod_switch<P>(tl0[n0]));
 #pragma unroll
-    for (int r = 0; r < 2 * R; r++) {
-        const int i = AccStore<P>::coef_index(r, lane);
-        acc.set(r, lane, j == 0 ? static_cast<T>(0) : test_vector_coef<P>(i, bar, mu));
+        for (int r = 0; r < 2 * R; r++) {
+            const int i = AccStore<P>::coef_index(r, lane);
+            acc.set(r, lane, j == 0 ? static_cast<T>(0) : test_vector_coef<P>(i, bar, mu));
+        }
+        if constexpr (AccStore<P>::kLds) wave_lds_sync();
     }
-    if constexpr (AccStore<P>::kLds) wave_lds_sync();
     constexpr uint32_t rnd = kMaskRound<P>;
+    auto abar = [&](int i) -> uint32_t {
+        if constexpr (GHOST) return 0u;
+        // the lvl0 TLWE is wave-uniform: keep the mod-switched values in SGPRs
+        return __builtin_amdgcn_readfirstlane(mod_switch<P>(tl0[i] + rnd));
+    };
     if constexpr (KB) {
         constexpr size_t step = 3 * kTrgswFourier<P>;
 #pragma unroll 1
         for (int it = 0; it < n0 / 2; it++) {
-            // the lvl0 TLWE is wave-uniform: keep the mod-switched values in SGPRs
-            const uint32_t a0 = __builtin_amdgcn_readfirstlane(mod_switch<P>(tl0[2 * it] + rnd));
-            const uint32_t a1 =
-                __builtin_amdgcn_readfirstlane(mod_switch<P>(tl0[2 * it + 1] + rnd));
-            br_step_fft<P, true>(acc, j, a0, a1, K.bk + it * step, my_spec, partner_spec, K);
+            const uint32_t a0 = abar(2 * it), a1 = abar(2 * it + 1);
+            br_step_fft<P, true, GHOST>(acc, j, a0, a1, GHOST ? nullptr : K.bk + it * step,
+                                        my_spec, partner_spec, K);
         }
     }
     else {
 #pragma unroll 1
         for (int it = 0; it < n0; it++) {
-            const uint32_t a = __builtin_amdgcn_readfirstlane(mod_switch<P>(tl0[it] + rnd));
-            br_step_fft<P, false>(acc, j, a, 0, K.bk + it * kTrgswFourier<P>, my_spec,
-                                  partner_spec, K);
+            br_step_fft<P, false, GHOST>(acc, j, abar(it), 0,
+                                         GHOST ? nullptr : K.bk + it * kTrgswFourier<P>, my_spec,
+                                         partner_spec, K);
         }
     }
 }
 
-// Workgroup barriers of one blind_rotate_fft<P, KB>: br_step_fft calls
-// __syncthreads() twice per digit pass, and blind_rotate_fft none of its
-// own. A wave of the workgroup that takes no part in the rotation calls
-// rotation_barriers<P, KB>() in its place to stay in step with the
-// rotating waves (every barrier counts all waves that have not ended).
+// The workgroup barriers of one blind_rotate_fft<P, KB> without its work:
+// a wave of the workgroup that takes no part in the rotation calls this in
+// its place to stay in step with the rotating waves.
 template <class P, bool KB>
-inline constexpr int kBrBarriers = (KB ? lvl0param::n / 2 : lvl0param::n) * BrShape<P>::kPasses * 2;
-
-template <class P, bool KB>
-__device__ __forceinline__ void rotation_barriers()
+__device__ __forceinline__ void blind_rotate_ghost(const FourierKeys<P>& K)
 {
-#pragma unroll 1
-    for (int i = 0; i < kBrBarriers<P, KB>; i++) __syncthreads();
+    AccStore<P> none;
+    blind_rotate_fft<P, KB, true>(nullptr, typename P::T(0), none, 0, nullptr, nullptr, K);
 }
 
 // The rotated TRLWE itself: component j's N coefficients at out + j N.

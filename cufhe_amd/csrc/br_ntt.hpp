@@ -71,7 +71,10 @@ __device__ __forceinline__ int ntt_coef_index(int r, int lane) { return lane + 6
 //   NTT_PLAIN: acc += ExtProd((X^a[0] - 1) acc, BK)            (trgsw0 = BK_i)
 //   NTT_KB   : acc  = ExtProd(acc, one + BK2 X(a1) + BK1 X(a0) + BK0 X(a0+a1))
 //   NTT_BB   : acc += ExtProd(acc, sum_l BK_l X(a_l)), l < ell  (trgsw0 = BK_block*ell)
-template <int MODE>
+// GHOST: the same loop nest and workgroup barriers with no loads, stores or
+// arithmetic (no pointer argument is dereferenced), for the waves of a
+// workgroup that do not rotate but must keep step with the ones that do.
+template <int MODE, bool GHOST = false>
 __device__ __forceinline__ void br_step_ntt(uint32_t (&acc)[16], int j, const uint32_t* a,
                                             const uint32_t* trgsw0, uint32_t* my_lds,
                                             const uint32_t* partner_lds, const NttKeys& K)
@@ -82,7 +85,9 @@ __device__ __forceinline__ void br_step_ntt(uint32_t (&acc)[16], int j, const ui
     const int lane = threadIdx.x & 63;
 
     uint32_t src[16];
-    if constexpr (MODE == NTT_PLAIN) {
+    if constexpr (GHOST) {
+    }
+    else if constexpr (MODE == NTT_PLAIN) {
 #pragma unroll
         for (int r = 0; r < 16; r++) my_lds[ntt_coef_index(r, lane)] = acc[r];
         wave_lds_sync();
@@ -102,7 +107,9 @@ __device__ __forceinline__ void br_step_ntt(uint32_t (&acc)[16], int j, const ui
     }
 
     XaiNtt xg[MODE == NTT_PLAIN ? 1 : NK];
-    if constexpr (MODE == NTT_KB) {
+    if constexpr (GHOST) {
+    }
+    else if constexpr (MODE == NTT_KB) {
         xg[0].init(a[0], K.tb.pw, lane);
         xg[1].init(a[1], K.tb.pw, lane);
         xg[2].init((a[0] + a[1]) & (2 * N - 1), K.tb.pw, lane);
@@ -111,7 +118,7 @@ __device__ __forceinline__ void br_step_ntt(uint32_t (&acc)[16], int j, const ui
 #pragma unroll
         for (int e = 0; e < NK; e++) xg[e].init(a[e], K.tb.pw, lane);
     }
-    const uint32_t* col0 = trgsw0 + static_cast<size_t>(j) * N + lane;
+    const uint32_t* col0 = GHOST ? nullptr : trgsw0 + static_cast<size_t>(j) * N + lane;
 
     uint32_t s[16];
 #pragma unroll
@@ -119,16 +126,18 @@ __device__ __forceinline__ void br_step_ntt(uint32_t (&acc)[16], int j, const ui
 
 #pragma unroll 1
     for (int q = 0; q < L; q++) {
-        if constexpr (MODE != NTT_PLAIN) {
+        if constexpr (MODE != NTT_PLAIN && !GHOST) {
 #pragma unroll
             for (int e = 0; e < NK; e++) xg[e].pin();
         }
-        uint32_t x[16];
+        if constexpr (!GHOST) {
+            uint32_t x[16];
 #pragma unroll
-        for (int r = 0; r < 16; r++) x[r] = signed_to_ntt31(Gadget<lvl1param>::digit(src[r], q));
-        ntt_forward(x, my_lds, K.tb);
+            for (int r = 0; r < 16; r++) x[r] = signed_to_ntt31(Gadget<lvl1param>::digit(src[r], q));
+            ntt_forward(x, my_lds, K.tb);
 #pragma unroll
-        for (int d = 0; d < 16; d++) my_lds[d * 64 + lane] = x[d];
+            for (int d = 0; d < 16; d++) my_lds[d * 64 + lane] = x[d];
+        }
         // key column j of rows (comp*L + q); keys [e] of the bundle / block
         uint32_t kv[2][2][NK];  // [slot][comp][e]
         auto load_keys = [&](int slot, int d) {
@@ -144,10 +153,10 @@ __device__ __forceinline__ void br_step_ntt(uint32_t (&acc)[16], int j, const ui
 #pragma unroll
         for (int qq = 0; qq < L; qq++)
             if (qq == q) one_q = K.one_mont[qq];
-        load_keys(0, 0);
+        if constexpr (!GHOST) load_keys(0, 0);
         __syncthreads();
 #pragma unroll
-        for (int d = 0; d < 16; d++) {
+        for (int d = 0; d < (GHOST ? 0 : 16); d++) {
             if (d + 1 < 16) load_keys((d + 1) & 1, d + 1);
             const int sl = d & 1;
             // products summed in pairs before one Montgomery reduction
@@ -179,6 +188,7 @@ __device__ __forceinline__ void br_step_ntt(uint32_t (&acc)[16], int j, const ui
         }
         __syncthreads();
     }
+    if constexpr (GHOST) return;
     ntt_inverse(s, my_lds, K.tb);
 #pragma unroll
     for (int r = 0; r < 16; r++) {
@@ -188,7 +198,9 @@ __device__ __forceinline__ void br_step_ntt(uint32_t (&acc)[16], int j, const ui
 }
 
 // Full rotation of the (pre-added) lvl0 TLWE tl0 (LDS) for the wave pair.
-template <int MODE>
+// GHOST: the loop nest and workgroup barriers only (see br_step_ntt); call
+// it as blind_rotate_ntt_ghost<MODE>().
+template <int MODE, bool GHOST = false>
 __device__ __forceinline__ void blind_rotate_ntt(const uint32_t* tl0, uint32_t mu, uint32_t (&acc)[16],
                                                  int j, uint32_t* my_lds, const uint32_t* partner_lds,
                                                  const NttKeys& K)
@@ -197,81 +209,82 @@ __device__ __forceinline__ void blind_rotate_ntt(const uint32_t* tl0, uint32_t m
     constexpr int N = P::n, n0 = lvl0param::n;
     constexpr uint32_t rnd = kMaskRound<P>;
     const int lane = threadIdx.x & 63;
-    uint32_t body = tl0[n0];
-    if constexpr (MODE == NTT_BB) {
-        // reference :814-838: correction = sum_i (a_i - round(a_i)), the body is
-        // shifted by correction/2 (signed, truncating) and rounded
-        constexpr int shift = 32 - 1 - P::nbit;
-        uint32_t part = 0;
-        for (int i = lane; i < n0; i += 64) {
-            const uint32_t moded = (tl0[i] + rnd) >> shift;
-            part += tl0[i] - (moded << shift);
-        }
+    if constexpr (!GHOST) {
+        uint32_t body = tl0[n0];
+        if constexpr (MODE == NTT_BB) {
+            // reference :814-838: correction = sum_i (a_i - round(a_i)), the body is
+            // shifted by correction/2 (signed, truncating) and rounded
+            constexpr int shift = 32 - 1 - P::nbit;
+            uint32_t part = 0;
+            for (int i = lane; i < n0; i += 64) {
+                const uint32_t moded = (tl0[i] + rnd) >> shift;
+                part += tl0[i] - (moded << shift);
+            }
 #pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) part += __shfl_xor(part, m);
-        const int32_t correction = static_cast<int32_t>(part);
-        body = body - static_cast<uint32_t>(correction / 2) + rnd;
-    }
-    const uint32_t bar = __builtin_amdgcn_readfirstlane(2 * N - mod_switch<P>(body));
+            for (int m = 32; m >= 1; m >>= 1) part += __shfl_xor(part, m);
+            const int32_t correction = static_cast<int32_t>(part);
+            body = body - static_cast<uint32_t>(correction / 2) + rnd;
+        }
+        const uint32_t bar = __builtin_amdgcn_readfirstlane(2 * N - mod_switch<P>(body));
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int i = ntt_coef_index(r, lane);
-        if (j == 0) {
-            acc[r] = 0;
-        }
-        else if (bar == 2 * N) {
-            acc[r] = mu;
-        }
-        else {
-            const bool neg = (static_cast<uint32_t>(i) < (bar & (N - 1))) ^ ((bar >> P::nbit) != 0);
-            acc[r] = neg ? 0u - mu : mu;
+        for (int r = 0; r < 16; r++) {
+            const int i = ntt_coef_index(r, lane);
+            if (j == 0) {
+                acc[r] = 0;
+            }
+            else if (bar == 2 * N) {
+                acc[r] = mu;
+            }
+            else {
+                const bool neg = (static_cast<uint32_t>(i) < (bar & (N - 1))) ^ ((bar >> P::nbit) != 0);
+                acc[r] = neg ? 0u - mu : mu;
+            }
         }
     }
+    auto abar = [&](int i) -> uint32_t {
+        if constexpr (GHOST) return 0u;
+        return __builtin_amdgcn_readfirstlane(mod_switch<P>(tl0[i] + rnd));
+    };
+    auto key = [&](size_t trgsw) -> const uint32_t* {
+        return GHOST ? nullptr : K.bk + trgsw * kTrgswNtt;
+    };
     uint32_t a[kBlockEll > 3 ? kBlockEll : 3];
     if constexpr (MODE == NTT_KB) {
 #pragma unroll 1
         for (int it = 0; it < n0 / 2; it++) {
-            a[0] = __builtin_amdgcn_readfirstlane(mod_switch<P>(tl0[2 * it] + rnd));
-            a[1] = __builtin_amdgcn_readfirstlane(mod_switch<P>(tl0[2 * it + 1] + rnd));
-            br_step_ntt<NTT_KB>(acc, j, a, K.bk + static_cast<size_t>(it) * 3 * kTrgswNtt, my_lds,
-                                partner_lds, K);
+            a[0] = abar(2 * it);
+            a[1] = abar(2 * it + 1);
+            br_step_ntt<NTT_KB, GHOST>(acc, j, a, key(static_cast<size_t>(it) * 3), my_lds,
+                                       partner_lds, K);
         }
     }
     else if constexpr (MODE == NTT_BB) {
 #pragma unroll 1
         for (int blk = 0; blk < n0 / kBlockEll; blk++) {
 #pragma unroll
-            for (int e = 0; e < kBlockEll; e++)
-                a[e] = __builtin_amdgcn_readfirstlane(mod_switch<P>(tl0[blk * kBlockEll + e] + rnd));
-            br_step_ntt<NTT_BB>(acc, j, a, K.bk + static_cast<size_t>(blk) * kBlockEll * kTrgswNtt,
-                                my_lds, partner_lds, K);
+            for (int e = 0; e < kBlockEll; e++) a[e] = abar(blk * kBlockEll + e);
+            br_step_ntt<NTT_BB, GHOST>(acc, j, a, key(static_cast<size_t>(blk) * kBlockEll), my_lds,
+                                       partner_lds, K);
         }
     }
     else {
 #pragma unroll 1
         for (int it = 0; it < n0; it++) {
-            a[0] = __builtin_amdgcn_readfirstlane(mod_switch<P>(tl0[it] + rnd));
-            br_step_ntt<NTT_PLAIN>(acc, j, a, K.bk + static_cast<size_t>(it) * kTrgswNtt, my_lds,
-                                   partner_lds, K);
+            a[0] = abar(it);
+            br_step_ntt<NTT_PLAIN, GHOST>(acc, j, a, key(static_cast<size_t>(it)), my_lds,
+                                          partner_lds, K);
         }
     }
 }
 
-// Workgroup barriers of one blind_rotate_ntt<MODE>: br_step_ntt calls
-// __syncthreads() twice per digit, and blind_rotate_ntt none of its own. A
-// wave of the workgroup that takes no part in the rotation calls
-// rotation_barriers_ntt<MODE>() in its place to stay in step with the
-// rotating waves (every barrier counts all waves that have not ended).
+// The workgroup barriers of one blind_rotate_ntt<MODE> without its work: a
+// wave of the workgroup that takes no part in the rotation calls this in its
+// place to stay in step with the rotating waves.
 template <int MODE>
-inline constexpr int kBrNttBarriers =
-    (MODE == NTT_KB ? lvl0param::n / 2 : (MODE == NTT_BB ? lvl0param::n / kBlockEll : lvl0param::n)) *
-    lvl1param::l * 2;
-
-template <int MODE>
-__device__ __forceinline__ void rotation_barriers_ntt()
+__device__ __forceinline__ void blind_rotate_ntt_ghost(const NttKeys& K)
 {
-#pragma unroll 1
-    for (int i = 0; i < kBrNttBarriers<MODE>; i++) __syncthreads();
+    uint32_t none[16];
+    blind_rotate_ntt<MODE, true>(nullptr, 0u, none, 0, nullptr, nullptr, K);
 }
 
 // The rotated TRLWE itself: component j's N coefficients at out + j N.

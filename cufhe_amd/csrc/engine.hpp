@@ -39,6 +39,8 @@ inline size_t lvl1_workspace_words(int nks, int nmux)
 // jobs, using ws (lvl1_workspace_words of this list) for intermediates.
 void lvl1_build(const cufhe_gate_desc* gates, int n, uint32_t* d_pool, uint32_t* ws, Lvl1Jobs& jobs);
 bool lvl1_loaded();
+// Changes whenever lvl1 keys are (re)loaded; 0 when none are loaded.
+uint64_t lvl1_key_generation();
 // Validated jobs-per-workgroup for nbr bootstraps (c = 0: the default for
 // the loaded keys and that batch size; nbr < 0: size unknown), or -1.
 int lvl1_pick_c(int c, int nbr);

@@ -276,7 +276,7 @@ __global__ __launch_bounds__(64 * C, 8 / C) void k_br_fft_kbw(const BrJob* __res
 template <bool KB, bool MUX, bool SUBSET>
 __global__ __launch_bounds__(512, 1) void k_gate_lvl01_fft(
     uint32_t* out, const uint32_t* in0, const uint32_t* in1, const uint32_t* inc, int32_t ca,
-    int32_t cb, uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
+    int32_t cb, uint32_t offset, int32_t mux_sign, uint32_t tv_mu, const uint32_t* ksk,
     FourierKeys<lvl1param> K)
 {
     using P = lvl1param;
@@ -310,11 +310,11 @@ __global__ __launch_bounds__(512, 1) void k_gate_lvl01_fft(
     __syncthreads();
     if (rotating) {
         AccStore<P> acc;
-        blind_rotate_fft<P, KB>(sh_tl0[slot], mu, acc, j, sh_spec[wave], sh_spec[wave ^ 1], K);
+        blind_rotate_fft<P, KB>(sh_tl0[slot], tv_mu, acc, j, sh_spec[wave], sh_spec[wave ^ 1], K);
         sample_extract_store<P>(acc, j, sh_res[slot]);
     }
     else {
-        rotation_barriers<P, KB>();
+        blind_rotate_ghost<P, KB>(K);
     }
     __syncthreads();
     if constexpr (MUX)
@@ -492,7 +492,7 @@ __global__ __launch_bounds__(512, 1) void k_gate_lvl1_kbw(
 template <bool MUX, bool SUBSET>
 __global__ __launch_bounds__(512, 1) void k_gate_lvl01_kbw(
     uint32_t* out, const uint32_t* in0, const uint32_t* in1, const uint32_t* inc, int32_t ca,
-    int32_t cb, uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
+    int32_t cb, uint32_t offset, int32_t mux_sign, uint32_t tv_mu, const uint32_t* ksk,
     FourierKeys<lvl1param> K)
 {
     using P = lvl1param;
@@ -532,7 +532,7 @@ __global__ __launch_bounds__(512, 1) void k_gate_lvl01_kbw(
     __syncthreads();
     if (slot < NJ) {
         uint32_t acc[2][16];
-        blind_rotate_kbw<C>(sm, slot, mu, acc, K.bk);
+        blind_rotate_kbw<C>(sm, slot, tv_mu, acc, K.bk);
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             AccStore<P> r;
@@ -608,7 +608,7 @@ __global__ __launch_bounds__(256, 1) void k_gate_lvl02_fft(
             sample_extract_store<P>(acc, j, sh_res[pass]);
         }
         else {
-            rotation_barriers<P, KB>();
+            blind_rotate_ghost<P, KB>(K);
         }
     }
     __syncthreads();
@@ -805,17 +805,17 @@ void launch_gate_lvl1_fft(bool kb, bool mux, uint32_t* out, const uint32_t* in0,
 
 void launch_gate_lvl01_fft(bool kb, bool mux, bool subset, uint32_t* out, const uint32_t* in0,
                            const uint32_t* in1, const uint32_t* inc, int32_t ca, int32_t cb,
-                           uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
+                           uint32_t offset, int32_t mux_sign, uint32_t tv_mu, const uint32_t* ksk,
                            const FourierKeys<lvl1param>& K, hipStream_t st)
 {
 #define CUFHE_G01(KBV, MUXV, SUB)                                                              \
     hipLaunchKernelGGL((k_gate_lvl01_fft<KBV, MUXV, SUB>), dim3(1), dim3(512), 0, \
-                       st, out, in0, in1, inc, ca, cb, offset, mux_sign, ksk, K)
+                       st, out, in0, in1, inc, ca, cb, offset, mux_sign, tv_mu, ksk, K)
     if (kb) {
         // key bundle: the one-wave rotation with seven key-streaming waves
 #define CUFHE_K01(MUXV, SUB)                                                                     \
     hipLaunchKernelGGL((k_gate_lvl01_kbw<MUXV, SUB>), dim3(1), dim3(512), 0, st, out, in0, in1, \
-                       inc, ca, cb, offset, mux_sign, ksk, K)
+                       inc, ca, cb, offset, mux_sign, tv_mu, ksk, K)
         if (mux) {
             if (subset) CUFHE_K01(true, true); else CUFHE_K01(true, false);
         }

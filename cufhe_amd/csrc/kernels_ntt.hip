@@ -182,7 +182,7 @@ __global__ __launch_bounds__(128 * C, 1) void k_br_ntt_shared(const BrJob* __res
 template <int MODE, bool SUBSET, bool MUX>
 __global__ __launch_bounds__(512, 1) void k_gate_lvl01_ntt(
     uint32_t* out, const uint32_t* in0, const uint32_t* in1, const uint32_t* inc, int32_t ca,
-    int32_t cb, uint32_t offset, int32_t mux_sign, const uint32_t* ksk, NttKeys K)
+    int32_t cb, uint32_t offset, int32_t mux_sign, uint32_t tv_mu, const uint32_t* ksk, NttKeys K)
 {
     using P = lvl1param;
     constexpr int C = MUX ? 2 : 1;
@@ -214,11 +214,11 @@ __global__ __launch_bounds__(512, 1) void k_gate_lvl01_ntt(
     __syncthreads();
     if (rotating) {
         uint32_t acc[16];
-        blind_rotate_ntt<MODE>(sh_tl0[slot], mu, acc, j, sh_lds[wave], sh_lds[wave ^ 1], K);
+        blind_rotate_ntt<MODE>(sh_tl0[slot], tv_mu, acc, j, sh_lds[wave], sh_lds[wave ^ 1], K);
         sample_extract_ntt(acc, j, sh_res[slot]);
     }
     else {
-        rotation_barriers_ntt<MODE>();
+        blind_rotate_ntt_ghost<MODE>(K);
     }
     __syncthreads();
     if constexpr (MUX)
@@ -374,12 +374,12 @@ void launch_gate_lvl1_ntt(int mode, bool subset, bool mux, uint32_t* out, const 
 
 void launch_gate_lvl01_ntt(int mode, bool subset, bool mux, uint32_t* out, const uint32_t* in0,
                            const uint32_t* in1, const uint32_t* inc, int32_t ca, int32_t cb,
-                           uint32_t offset, int32_t mux_sign, const uint32_t* ksk, const NttKeys& K,
-                           hipStream_t st)
+                           uint32_t offset, int32_t mux_sign, uint32_t tv_mu, const uint32_t* ksk,
+                           const NttKeys& K, hipStream_t st)
 {
 #define CUFHE_G01N(M, S, X)                                                                     \
     hipLaunchKernelGGL((k_gate_lvl01_ntt<M, S, X>), dim3(1), dim3(512), 0, st, out, in0,          \
-                       in1, inc, ca, cb, offset, mux_sign, ksk, K)
+                       in1, inc, ca, cb, offset, mux_sign, tv_mu, ksk, K)
 #define CUFHE_G01N_MODE(M)                                                \
     if (subset) {                                                         \
         if (mux) CUFHE_G01N(M, true, true); else CUFHE_G01N(M, true, false); \

@@ -30,9 +30,12 @@ void launch_gate_lvl1_fft(bool kb, bool mux, uint32_t* out, const uint32_t* in0,
                           const uint32_t* in1, const uint32_t* inc, int32_t ca, int32_t cb,
                           uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
                           const FourierKeys<lvl1param>& K, hipStream_t st);
+// lvl0 API gate (BR-IKS at lvl1). tv_mu: the test vector's amplitude (the
+// gate's mu; the reference Bootstrap() passes its own); MUX/NMUX combine
+// with lvl1param's mu.
 void launch_gate_lvl01_fft(bool kb, bool mux, bool subset, uint32_t* out, const uint32_t* in0,
                            const uint32_t* in1, const uint32_t* inc, int32_t ca, int32_t cb,
-                           uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
+                           uint32_t offset, int32_t mux_sign, uint32_t tv_mu, const uint32_t* ksk,
                            const FourierKeys<lvl1param>& K, hipStream_t st);
 void launch_se_ks_lvl10(bool subset, uint32_t* out, const uint32_t* trlwe, int n,
                         const uint32_t* ksk, hipStream_t st);
@@ -57,8 +60,8 @@ void launch_br_ntt(int mode, const BrJob* jobs, int njobs, int c, uint32_t mu, c
                    hipStream_t st);
 void launch_gate_lvl01_ntt(int mode, bool subset, bool mux, uint32_t* out, const uint32_t* in0,
                            const uint32_t* in1, const uint32_t* inc, int32_t ca, int32_t cb,
-                           uint32_t offset, int32_t mux_sign, const uint32_t* ksk, const NttKeys& K,
-                           hipStream_t st);
+                           uint32_t offset, int32_t mux_sign, uint32_t tv_mu, const uint32_t* ksk,
+                           const NttKeys& K, hipStream_t st);
 void launch_gate_lvl1_ntt(int mode, bool subset, bool mux, uint32_t* out, const uint32_t* in0,
                           const uint32_t* in1, const uint32_t* inc, int32_t ca, int32_t cb,
                           uint32_t offset, int32_t mux_sign, const uint32_t* ksk, const NttKeys& K,

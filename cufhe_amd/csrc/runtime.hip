@@ -56,6 +56,7 @@ struct NttTableSet {
 template <class P>
 struct LevelKeys {
     bool loaded = false;
+    uint64_t generation = 0;  // changes on every load: plans and servers check it
     bool kb = false;
     int backend = CUFHE_BACKEND_FFT;
     int variant = CUFHE_BR_PLAIN;
@@ -71,6 +72,7 @@ struct LevelKeys {
 struct State {
     std::mutex mu;
     bool initialized = false;
+    uint64_t generations = 0;  // key loads so far
     int device = 0;
     TableSet<lvl1param> t1;
     TableSet<lvl2param> t2;
@@ -490,6 +492,12 @@ void lvl1_build(const cufhe_gate_desc* gates, int n, uint32_t* d_pool, uint32_t*
 
 bool lvl1_loaded() { return state().k1.loaded; }
 
+uint64_t lvl1_key_generation()
+{
+    const State& s = state();
+    return s.k1.loaded ? s.k1.generation : 0;
+}
+
 int lvl1_pick_c(int c, int nbr)
 {
     const LevelKeys<lvl1param>& k = state().k1;
@@ -541,6 +549,7 @@ using namespace cufhe_amd;
 
 struct cufhe_plan {
     int level = 1;  // 1: lvl1 gates (IKS-BR); 2: lvl02 gates (BR-IKS); 3: lvl0 gates (BR-IKS at lvl1)
+    uint64_t key_generation = 0;  // the key load the plan was built for
     int njobs_ks = 0, njobs_br = 0, njobs_lin = 0;
     int c = 2;
     bool use_graph = false;
@@ -560,6 +569,7 @@ namespace {
 int plan_alloc_ks_ws(cufhe_plan* p)
 {
     State& s = state();
+    p->key_generation = p->level == 2 ? s.k2.generation : s.k1.generation;
     const KsGemmKey& key = p->level == 2 ? s.k2.ksg : s.k1.ksg;
     if (p->njobs_ks < kKsGemmMinJobs || !key.planes) return CUFHE_OK;
     if (hipMalloc(&p->ks_ws, ks_gemm_workspace_bytes(key, p->njobs_ks)) != hipSuccess)
@@ -596,6 +606,24 @@ int plan_launch(cufhe_plan* p, hipStream_t st)
     if (!lvl1_launch(p->ks, p->njobs_ks, p->ks_ws, p->br, p->njobs_br, p->lin, p->njobs_lin, p->c, st))
         return fail(CUFHE_ERR_HIP, "GEMM key switch failed");
     return check_launch("batched gate launch");
+}
+
+// The plan's kernels, job arrays and workspace were sized for the keys loaded
+// when it was created; a reload (even of the same sizes) invalidates it.
+int plan_check_keys(const cufhe_plan* p)
+{
+    const State& s = state();
+    const LevelKeys<lvl1param>& k1 = s.k1;
+    if (p->level == 2) {
+        if (!s.k2.loaded) return fail(CUFHE_ERR_STATE, "lvl02 keys not loaded");
+        if (s.k2.generation != p->key_generation)
+            return fail(CUFHE_ERR_STATE, "lvl02 keys were reloaded after the plan was created");
+        return CUFHE_OK;
+    }
+    if (!k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded");
+    if (k1.generation != p->key_generation)
+        return fail(CUFHE_ERR_STATE, "lvl1 keys were reloaded after the plan was created");
+    return CUFHE_OK;
 }
 
 int pick_c(int c, bool kb, int nbr)
@@ -667,6 +695,7 @@ int cufhe_initialize_lvl01_ex(const uint32_t* bk, size_t bk_words, int backend, 
     s.k1.variant = variant;
     s.k1.kb = variant == CUFHE_BR_KEY_BUNDLE;
     s.k1.subset = subset_ksk != 0;
+    s.k1.generation = ++s.generations;
     s.k1.loaded = true;
     return CUFHE_OK;
 }
@@ -688,6 +717,7 @@ int cufhe_initialize_lvl02(const uint64_t* bk, size_t bk_words, int key_bundle,
         free_keys(s.k2);
         return rc;
     }
+    s.k2.generation = ++s.generations;
     s.k2.loaded = true;
     return CUFHE_OK;
 }
@@ -1009,13 +1039,32 @@ int cufhe_gate_lvl0(int op, uint32_t* out, const uint32_t* in0, const uint32_t* 
     }
     const uint32_t off = static_cast<uint32_t>(gl.off_mu) * mu0;
     const int sign = op == CUFHE_NMUX ? -1 : 1;
+    constexpr uint32_t mu1 = static_cast<uint32_t>(lvl1param::μ);
     if (k.backend == CUFHE_BACKEND_NTT)
         launch_gate_lvl01_ntt(k.variant, k.subset, mux, out, in0, in1, inc, gl.ca, gl.cb, off, sign,
+                              mu1, k.ksk, k.nk, st);
+    else
+        launch_gate_lvl01_fft(k.kb, mux, k.subset, out, in0, in1, inc, gl.ca, gl.cb, off, sign, mu1,
+                              k.ksk, k.fk, st);
+    return check_launch("lvl0 gate");
+}
+
+int cufhe_bootstrap(uint32_t* d_out, const uint32_t* d_in, uint32_t mu, void* stream)
+{
+    State& s = state();
+    if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded (cufhe_initialize_lvl01)");
+    if (!d_out || !d_in) return fail(CUFHE_ERR_ARG, "null ciphertext pointer");
+    const LevelKeys<lvl1param>& k = s.k1;
+    const hipStream_t st = as_stream(stream);
+    // the lvl0 gate kernel with (ca, cb, offset) = (1, 0, 0): rotation of the
+    // input itself with test vector +-mu, SampleExtract, key switch to lvl0
+    if (k.backend == CUFHE_BACKEND_NTT)
+        launch_gate_lvl01_ntt(k.variant, k.subset, false, d_out, d_in, d_in, nullptr, 1, 0, 0u, 1, mu,
                               k.ksk, k.nk, st);
     else
-        launch_gate_lvl01_fft(k.kb, mux, k.subset, out, in0, in1, inc, gl.ca, gl.cb, off, sign, k.ksk,
-                              k.fk, st);
-    return check_launch("lvl0 gate");
+        launch_gate_lvl01_fft(k.kb, false, k.subset, d_out, d_in, d_in, nullptr, 1, 0, 0u, 1, mu,
+                              k.ksk, k.fk, st);
+    return check_launch("bootstrap");
 }
 
 int cufhe_sample_extract_key_switch_lvl10(uint32_t* d_out, const uint32_t* d_trlwe, int n,
@@ -1044,7 +1093,8 @@ int cufhe_bootstrap_to_trlwe_lvl01(uint32_t* d_out, const uint32_t* d_in, uint32
     BrJob* d = nullptr;
     CUFHE_HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&d), n * sizeof(BrJob), st));
     CUFHE_HIP_TRY(hipMemcpyAsync(d, jobs.data(), n * sizeof(BrJob), hipMemcpyHostToDevice, st));
-    br_lvl1(d, n, 1, mu, st);
+    // the default rotation kernel of the loaded keys for this batch size
+    br_lvl1(d, n, lvl1_pick_c(0, n), mu, st);
     CUFHE_HIP_TRY(hipFreeAsync(d, st));
     CUFHE_HIP_TRY(hipStreamSynchronize(st));
     return check_launch("bootstrap to TRLWE");
@@ -1151,11 +1201,8 @@ int cufhe_plan_create_lvl02(const cufhe_gate_desc* gates, int n, uint32_t* d_poo
 int cufhe_plan_run(cufhe_plan* p, void* stream)
 {
     if (!p) return fail(CUFHE_ERR_ARG, "null plan");
-    State& s = state();
-    if (!(p->level == 2 ? s.k2.loaded : s.k1.loaded))
-        return fail(CUFHE_ERR_STATE, "keys of the plan's level not loaded");
-    if (p->level != 2 && p->c != lvl1_pick_c(p->c, -1))
-        return fail(CUFHE_ERR_STATE, "jobs_per_group of the plan does not fit the loaded lvl1 keys");
+    const int krc = plan_check_keys(p);
+    if (krc != CUFHE_OK) return krc;
     const hipStream_t st = as_stream(stream);
     if (!p->use_graph) return plan_launch(p, st);
     if (!p->exec) {
@@ -1197,12 +1244,22 @@ int cufhe_plan_jobs_per_group(const cufhe_plan* p) { return p ? p->c : 0; }
 int cufhe_plan_run_timed(cufhe_plan* p, void* stream, float* kernel_ms)
 {
     if (!p || !kernel_ms) return fail(CUFHE_ERR_ARG, "null argument");
+    const int krc = plan_check_keys(p);
+    if (krc != CUFHE_OK) return krc;
     State& s = state();
     const hipStream_t st = as_stream(stream);
+    // events live until the function returns, on every path
+    struct Events {
+        hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+        ~Events()
+        {
+            for (auto& x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } evs;
+    for (auto& x : evs.e) CUFHE_HIP_TRY(hipEventCreate(&x));
+    hipEvent_t* e = evs.e;
     if (p->level == 2) {
-        if (!s.k2.loaded) return fail(CUFHE_ERR_STATE, "lvl02 keys not loaded");
-        hipEvent_t e[4];
-        for (auto& x : e) CUFHE_HIP_TRY(hipEventCreate(&x));
         CUFHE_HIP_TRY(hipEventRecord(e[0], st));
         if (p->njobs_br) {
             if (s.k2.kb)
@@ -1213,7 +1270,8 @@ int cufhe_plan_run_timed(cufhe_plan* p, void* stream, float* kernel_ms)
                                                 static_cast<uint64_t>(lvl2param::μ), s.k2.fk, st);
         }
         CUFHE_HIP_TRY(hipEventRecord(e[1], st));
-        (void)ks_lvl2(p->ks, p->njobs_ks, p->ks_ws, st);
+        if (!ks_lvl2(p->ks, p->njobs_ks, p->ks_ws, st))
+            return fail(CUFHE_ERR_HIP, "key switch failed to launch");
         CUFHE_HIP_TRY(hipEventRecord(e[2], st));
         if (p->njobs_lin) launch_linear(p->lin, p->njobs_lin, st);
         CUFHE_HIP_TRY(hipEventRecord(e[3], st));
@@ -1228,23 +1286,20 @@ int cufhe_plan_run_timed(cufhe_plan* p, void* stream, float* kernel_ms)
             (void)hipEventElapsedTime(&kernel_ms[0], e[1], e[2]);
             (void)hipEventElapsedTime(&kernel_ms[2], e[2], e[3]);
         }
-        for (auto& x : e) (void)hipEventDestroy(x);
         return rc2;
     }
-    if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded");
-    if (p->c != lvl1_pick_c(p->c, -1))
-        return fail(CUFHE_ERR_STATE, "jobs_per_group of the plan does not fit the loaded lvl1 keys");
-    hipEvent_t ev[4];
-    for (auto& e : ev) CUFHE_HIP_TRY(hipEventCreate(&e));
+    hipEvent_t* ev = e;
     CUFHE_HIP_TRY(hipEventRecord(ev[0], st));
     if (p->level == 3) {
         // BR-IKS order; reported as [0] key switch, [1] rotation, [2] combine
         br_lvl1(p->br, p->njobs_br, p->c, static_cast<uint32_t>(lvl1param::μ), st);
         CUFHE_HIP_TRY(hipEventRecord(ev[1], st));
-        (void)ks_lvl1(p->ks, p->njobs_ks, p->ks_ws, st);
+        if (!ks_lvl1(p->ks, p->njobs_ks, p->ks_ws, st))
+            return fail(CUFHE_ERR_HIP, "key switch failed to launch");
     }
     else {
-        (void)ks_lvl1(p->ks, p->njobs_ks, p->ks_ws, st);
+        if (!ks_lvl1(p->ks, p->njobs_ks, p->ks_ws, st))
+            return fail(CUFHE_ERR_HIP, "key switch failed to launch");
         CUFHE_HIP_TRY(hipEventRecord(ev[1], st));
         br_lvl1(p->br, p->njobs_br, p->c, static_cast<uint32_t>(lvl1param::μ), st);
     }
@@ -1260,7 +1315,6 @@ int cufhe_plan_run_timed(cufhe_plan* p, void* stream, float* kernel_ms)
         for (int i = 0; i < 3; i++) (void)hipEventElapsedTime(&kernel_ms[i], ev[i], ev[i + 1]);
         if (p->level == 3) std::swap(kernel_ms[0], kernel_ms[1]);
     }
-    for (auto& e : ev) (void)hipEventDestroy(e);
     return rc;
 }
 
@@ -1297,6 +1351,26 @@ int cufhe_identity_key_switch_lvl10(uint32_t* d_out, const uint32_t* d_in, int n
     CUFHE_HIP_TRY(hipFreeAsync(d, st));
     CUFHE_HIP_TRY(hipStreamSynchronize(st));
     return check_launch("identity key switch");
+}
+
+int cufhe_identity_key_switch_lvl20(uint32_t* d_out, const uint64_t* d_in, int n, void* stream)
+{
+    State& s = state();
+    if (!s.k2.loaded) return fail(CUFHE_ERR_STATE, "lvl02 keys not loaded");
+    if (n <= 0) return n == 0 ? CUFHE_OK : fail(CUFHE_ERR_ARG, "negative count");
+    if (!d_out || !d_in) return fail(CUFHE_ERR_ARG, "null pointer");
+    const hipStream_t st = as_stream(stream);
+    std::vector<KsJob> jobs(n);
+    for (int i = 0; i < n; i++)
+        jobs[i] = KsJob{d_in + static_cast<size_t>(i) * (lvl2param::n + 1), nullptr,
+                        d_out + static_cast<size_t>(i) * (lvl0param::n + 1), 1, 0, 0};
+    KsJob* d = nullptr;
+    CUFHE_HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&d), n * sizeof(KsJob), st));
+    CUFHE_HIP_TRY(hipMemcpyAsync(d, jobs.data(), n * sizeof(KsJob), hipMemcpyHostToDevice, st));
+    (void)ks_lvl2(d, n, nullptr, st);
+    CUFHE_HIP_TRY(hipFreeAsync(d, st));
+    CUFHE_HIP_TRY(hipStreamSynchronize(st));
+    return check_launch("identity key switch lvl20");
 }
 
 int cufhe_bootstrap_lvl01(uint32_t* d_out, const uint32_t* d_in, uint32_t mu, int n,

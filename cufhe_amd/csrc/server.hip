@@ -105,6 +105,7 @@ struct cufhe_server {
     size_t pool_wires = 0;
     WireArena arena;
     int max_boot = 0, c = 1;
+    uint64_t key_generation = 0;  // the lvl1 key load the workspaces were sized for
     hipStream_t st = nullptr;
     KsJob* d_ks = nullptr;
     BrJob* d_br = nullptr;
@@ -195,6 +196,11 @@ void worker_loop(cufhe_server* S)
             S->stats[2] += ng;
         }
         const double tb = S->now();
+        if (lvl1_key_generation() != S->key_generation) {
+            std::lock_guard<std::mutex> lk(S->mu);
+            fail_all_locked(S, CUFHE_ERR_STATE, "lvl1 keys were reloaded while the server ran");
+            continue;
+        }
         hipError_t e = hipSuccess;
         for (auto& r : admitted_now) {
             if (e != hipSuccess) break;
@@ -303,6 +309,7 @@ int cufhe_server_create(size_t pool_bytes, int max_batch_bootstraps, int jobs_pe
         return set_error(CUFHE_ERR_NOMEM, std::string("server allocation: ") + hipGetErrorString(e));
     }
     S->arena.reset(S->pool_wires);
+    S->key_generation = lvl1_key_generation();
     S->t0 = Clock::now();
     S->worker = std::thread(worker_loop, S);
     *server = S;

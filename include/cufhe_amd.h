@@ -166,6 +166,11 @@ int cufhe_plan_create_lvl1(const cufhe_gate_desc* gates, int n, uint32_t* d_pool
  * TLWEs (wire w at d_pool + 637 w); same semantics as the lvl02 gates. */
 int cufhe_plan_create_lvl02(const cufhe_gate_desc* gates, int n, uint32_t* d_pool,
                             int use_graph, cufhe_plan** plan);
+/* A plan is bound to the keys loaded when it was created (its kernels, job
+ * arrays and key-switch workspace are sized for them): after any
+ * cufhe_initialize_lvl01* / _lvl02 of its level, cufhe_plan_run and
+ * cufhe_plan_run_timed return CUFHE_ERR_STATE without launching; create
+ * the plan again. */
 int cufhe_plan_run(cufhe_plan* plan, void* stream);
 int cufhe_plan_destroy(cufhe_plan* plan);
 /* Bootstraps (blind rotations) a plan performs per run. */
@@ -189,8 +194,9 @@ int cufhe_gate_batch_lvl1(const cufhe_gate_desc* gates, int n, uint32_t* d_pool,
  * bootstraps; a layer is never split), and circuits that arrive meanwhile
  * join the following round. Results are bit-identical to running each
  * circuit alone (every gate is computed independently of its batch).
- * Uses the lvl1 keys loaded at submit time; do not reload keys while a
- * server has work. */
+ * Uses the lvl1 keys loaded when the server was created: after a reload
+ * every pending and later request fails with CUFHE_ERR_STATE (create a new
+ * server). */
 typedef struct cufhe_server cufhe_server;
 typedef struct {
     int n_inputs;                  /* wires [0, n_inputs) are the inputs */
@@ -225,13 +231,28 @@ int cufhe_plan_create_lvl01(const cufhe_gate_desc* gates, int n, uint32_t* d_poo
                             int jobs_per_group, int use_graph, cufhe_plan** plan);
 
 /* ---- components ----------------------------------------------------------
- * IdentityKeySwitch lvl1 -> lvl0 of n ciphertexts (reference
- * include/keyswitch_gpu.cuh:128-187 KeySwitchFromTLWE / SEIandKS
- * src/keyswitch_gpu.cu:61-68). in: n x 1025, out: n x 637 words. */
+ * IdentityKeySwitch lvl1 -> lvl0 of n extracted TLWEs (reference
+ * include/keyswitch_gpu.cuh:128-187 KeySwitchFromTLWE<lvl10param>).
+ * in: n x 1025, out: n x 637 words. */
 int cufhe_identity_key_switch_lvl10(uint32_t* d_out, const uint32_t* d_in, int n, void* stream);
-/* Bootstrap: blind rotation of n lvl0 TLWEs with test vector +-mu and
- * sample extraction to lvl1 (reference src/bootstrap_gpu.cu:2346-2355).
- * in: n x 637, out: n x 1025 words. */
+/* IdentityKeySwitch lvl2 -> lvl0 with the lvl02 keys (reference
+ * include/keyswitch_gpu.cuh:128-187 KeySwitchFromTLWE<lvl20param>, the body
+ * rounded from Torus64 to Torus32 as KeySwitchTorusConvert :33-50; its GPU
+ * diagnostic test/test_keyswitch_diag.cu:31-127). in: n x 2049 uint64,
+ * out: n x 637 uint32 words. */
+int cufhe_identity_key_switch_lvl20(uint32_t* d_out, const uint64_t* d_in, int n, void* stream);
+/* Bootstrap(out, in, mu, st, gpu) (reference include/bootstrap_gpu.cuh:61-64,
+ * src/bootstrap_gpu.cu:1045-1058 and :2346-2355): blind rotation of ONE lvl0
+ * TLWE at lvl1 with test vector +-mu, SampleExtract and key switch back to
+ * lvl0 (the subset key when loaded so): 637 words in and out, one launch on
+ * `stream`, asynchronous like the reference. out may alias in. (The
+ * reference launches its kernel with no dynamic LDS and always the plain
+ * rotation; here the loaded keys' rotation runs.) */
+int cufhe_bootstrap(uint32_t* d_out, const uint32_t* d_in, uint32_t mu, void* stream);
+/* Blind rotation of n lvl0 TLWEs with test vector +-mu and sample
+ * extraction to lvl1, without the key switch that Bootstrap() adds (the
+ * rotation + SampleExtract half of the reference's BR-IKS gates,
+ * src/bootstrap_gpu.cu:1248-1273). in: n x 637, out: n x 1025 words. */
 int cufhe_bootstrap_lvl01(uint32_t* d_out, const uint32_t* d_in, uint32_t mu, int n,
                           int jobs_per_group, void* stream);
 /* TRLWE extras (reference src/cufhe_gates_gpu.cu:134-196): n lvl1 TRLWEs

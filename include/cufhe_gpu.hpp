@@ -279,6 +279,179 @@ void gCopy(Ctxt<P>& out, Ctxt<P>& in, Stream st)
     detail::gate<P>(CUFHE_COPY, out, &in, nullptr, nullptr, st);
 }
 
+// ---- lower launcher layer (reference include/bootstrap_gpu.cuh:40-217,
+// include/keyswitch_gpu.cuh:258-275): raw device pointers, a stream and the
+// GPU index (0: one GPU per process). Asynchronous on st like the reference.
+namespace detail {
+inline void check_gpu(int gpuNum)
+{
+    if (gpuNum != 0) {
+        std::fprintf(stderr, "cufhe: gpuNum %d: one GPU per process (index 0)\n", gpuNum);
+        std::exit(-1);
+    }
+}
+// BR-IKS order: lvl0 in and out, rotation at brP's target level.
+template <class brP, class iksP>
+inline void br_iks(int op, typename iksP::targetP::T* out, const typename brP::domainP::T* in0,
+                   const typename brP::domainP::T* in1, const typename brP::domainP::T* inc,
+                   hipStream_t st, int gpuNum)
+{
+    check_gpu(gpuNum);
+    static_assert(std::is_same_v<typename brP::domainP, TFHEpp::lvl0param> &&
+                      std::is_same_v<typename iksP::targetP, TFHEpp::lvl0param> &&
+                      std::is_same_v<typename brP::targetP, typename iksP::domainP>,
+                  "BR-IKS gates: lvl0 -> lvl1 -> lvl0 or lvl0 -> lvl2 -> lvl0");
+    auto* fn = std::is_same_v<brP, TFHEpp::lvl01param> ? cufhe_gate_lvl0 : cufhe_gate_lvl02;
+    CuCheck(fn(op, out, in0, in1, inc, st), "XxxBootstrap (BR-IKS)");
+}
+// IKS-BR order: lvl1 in and out.
+template <class iksP, class brP>
+inline void iks_br(int op, typename brP::targetP::T* out, const typename iksP::domainP::T* in0,
+                   const typename iksP::domainP::T* in1, const typename iksP::domainP::T* inc,
+                   hipStream_t st, int gpuNum)
+{
+    check_gpu(gpuNum);
+    static_assert(std::is_same_v<iksP, TFHEpp::lvl10param> &&
+                      std::is_same_v<brP, TFHEpp::lvl01param>,
+                  "IKS-BR gates: lvl1 -> lvl0 -> lvl1");
+    CuCheck(cufhe_gate_lvl1(op, out, in0, in1, inc, st), "XxxBootstrap (IKS-BR)");
+}
+}  // namespace detail
+
+// XxxBootstrap<brP, mu, iksP>(out, in0, in1, st, gpu): BR-IKS (lvl0 in/out);
+// XxxBootstrap<iksP, brP, mu>(out, in0, in1, st, gpu): IKS-BR (lvl1 in/out).
+// mu must be the target level's mu, the only value the reference's gate API
+// passes (src/cufhe_gates_gpu.cu:206-977).
+#define CUFHE_BOOTSTRAP_LAUNCHER(Name, Code)                                                     \
+    template <class brP, typename brP::targetP::T μ, class iksP>                                 \
+    void Name##Bootstrap(typename iksP::targetP::T* const out,                                  \
+                         const typename brP::domainP::T* const in0,                             \
+                         const typename brP::domainP::T* const in1, const hipStream_t st,       \
+                         const int gpuNum)                                                      \
+    {                                                                                           \
+        static_assert(μ == static_cast<typename brP::targetP::T>(brP::targetP::μ),             \
+                      "test vector mu must be the target level's mu");                          \
+        detail::br_iks<brP, iksP>(Code, out, in0, in1, nullptr, st, gpuNum);                     \
+    }                                                                                           \
+    template <class iksP, class brP, typename brP::targetP::T μ>                                 \
+    void Name##Bootstrap(typename brP::targetP::T* const out,                                   \
+                         const typename iksP::domainP::T* const in0,                            \
+                         const typename iksP::domainP::T* const in1, const hipStream_t st,      \
+                         const int gpuNum)                                                      \
+    {                                                                                           \
+        static_assert(μ == static_cast<typename brP::targetP::T>(brP::targetP::μ),             \
+                      "test vector mu must be the target level's mu");                          \
+        detail::iks_br<iksP, brP>(Code, out, in0, in1, nullptr, st, gpuNum);                     \
+    }
+CUFHE_BOOTSTRAP_LAUNCHER(Nand, CUFHE_NAND)
+CUFHE_BOOTSTRAP_LAUNCHER(Or, CUFHE_OR)
+CUFHE_BOOTSTRAP_LAUNCHER(OrYN, CUFHE_ORYN)
+CUFHE_BOOTSTRAP_LAUNCHER(OrNY, CUFHE_ORNY)
+CUFHE_BOOTSTRAP_LAUNCHER(And, CUFHE_AND)
+CUFHE_BOOTSTRAP_LAUNCHER(AndYN, CUFHE_ANDYN)
+CUFHE_BOOTSTRAP_LAUNCHER(AndNY, CUFHE_ANDNY)
+CUFHE_BOOTSTRAP_LAUNCHER(Nor, CUFHE_NOR)
+CUFHE_BOOTSTRAP_LAUNCHER(Xor, CUFHE_XOR)
+CUFHE_BOOTSTRAP_LAUNCHER(Xnor, CUFHE_XNOR)
+#undef CUFHE_BOOTSTRAP_LAUNCHER
+
+// MuxBootstrap / NMuxBootstrap: out = inc ? in1 : in0 (negated for NMux).
+#define CUFHE_MUX_LAUNCHER(Name, Code)                                                           \
+    template <class brP, typename brP::targetP::T μ, class iksP>                                 \
+    void Name##Bootstrap(typename iksP::targetP::T* const out,                                  \
+                         const typename brP::domainP::T* const inc,                             \
+                         const typename brP::domainP::T* const in1,                             \
+                         const typename brP::domainP::T* const in0, const hipStream_t st,       \
+                         const int gpuNum)                                                      \
+    {                                                                                           \
+        static_assert(μ == static_cast<typename brP::targetP::T>(brP::targetP::μ),             \
+                      "test vector mu must be the target level's mu");                          \
+        detail::br_iks<brP, iksP>(Code, out, in0, in1, inc, st, gpuNum);                         \
+    }                                                                                           \
+    template <class iksP, class brP, typename brP::targetP::T μ>                                 \
+    void Name##Bootstrap(typename brP::targetP::T* const out,                                   \
+                         const typename iksP::domainP::T* const inc,                            \
+                         const typename iksP::domainP::T* const in1,                            \
+                         const typename iksP::domainP::T* const in0, const hipStream_t st,      \
+                         const int gpuNum)                                                      \
+    {                                                                                           \
+        static_assert(μ == static_cast<typename brP::targetP::T>(brP::targetP::μ),             \
+                      "test vector mu must be the target level's mu");                          \
+        detail::iks_br<iksP, brP>(Code, out, in0, in1, inc, st, gpuNum);                         \
+    }
+CUFHE_MUX_LAUNCHER(Mux, CUFHE_MUX)
+CUFHE_MUX_LAUNCHER(NMux, CUFHE_NMUX)
+#undef CUFHE_MUX_LAUNCHER
+
+// CopyBootstrap<P> / NotBootstrap<P>: coefficient-wise (reference
+// src/bootstrap_gpu.cu:2200-2222, :3607-3640).
+template <class P>
+void CopyBootstrap(typename P::T* const out, const typename P::T* const in, const hipStream_t st,
+                   const int gpuNum)
+{
+    detail::check_gpu(gpuNum);
+    static_assert(std::is_same_v<P, TFHEpp::lvl0param> || std::is_same_v<P, TFHEpp::lvl1param>,
+                  "CopyBootstrap: lvl0 or lvl1 TLWEs");
+    auto* fn = std::is_same_v<P, TFHEpp::lvl1param> ? cufhe_gate_lvl1 : cufhe_gate_lvl0;
+    CuCheck(fn(CUFHE_COPY, out, in, nullptr, nullptr, st), "CopyBootstrap");
+}
+template <class P>
+void NotBootstrap(typename P::T* const out, const typename P::T* const in, const hipStream_t st,
+                  const int gpuNum)
+{
+    detail::check_gpu(gpuNum);
+    static_assert(std::is_same_v<P, TFHEpp::lvl0param> || std::is_same_v<P, TFHEpp::lvl1param>,
+                  "NotBootstrap: lvl0 or lvl1 TLWEs");
+    auto* fn = std::is_same_v<P, TFHEpp::lvl1param> ? cufhe_gate_lvl1 : cufhe_gate_lvl0;
+    CuCheck(fn(CUFHE_NOT, out, in, nullptr, nullptr, st), "NotBootstrap");
+}
+
+// Bootstrap(out, in, mu, st, gpu) (reference include/bootstrap_gpu.cuh:61-64,
+// src/bootstrap_gpu.cu:1045-1058): lvl0 -> blind rotation with test vector
+// +-mu at lvl1 -> SampleExtract -> key switch -> lvl0.
+inline void Bootstrap(TFHEpp::lvl0param::T* const out, const TFHEpp::lvl0param::T* const in,
+                      const TFHEpp::lvl1param::T mu, const hipStream_t st, const int gpuNum)
+{
+    detail::check_gpu(gpuNum);
+    CuCheck(cufhe_bootstrap(out, in, mu, st), "Bootstrap");
+}
+
+// BootstrapTLWE2TRLWE(out, in, mu, st, gpu) (reference
+// src/bootstrap_gpu.cu:2371-2388): the rotated test-vector TRLWE, no extraction.
+inline void BootstrapTLWE2TRLWE(TFHEpp::lvl1param::T* const out,
+                                const TFHEpp::lvl0param::T* const in,
+                                const TFHEpp::lvl1param::T mu, const hipStream_t st,
+                                const int gpuNum)
+{
+    detail::check_gpu(gpuNum);
+    CuCheck(cufhe_bootstrap_to_trlwe_lvl01(out, in, mu, 1, st), "BootstrapTLWE2TRLWE");
+}
+
+// SEIandKS(out, in, st, gpu) (reference include/keyswitch_gpu.cuh:273-275,
+// src/keyswitch_gpu.cu:61-68): SampleExtract index 0 of a lvl1 TRLWE and key
+// switch to lvl0.
+inline void SEIandKS(TFHEpp::lvl0param::T* const out, const TFHEpp::lvl1param::T* const in,
+                     const hipStream_t& st, const int gpuNum)
+{
+    detail::check_gpu(gpuNum);
+    CuCheck(cufhe_sample_extract_key_switch_lvl10(out, in, 1, st), "SEIandKS");
+}
+
+// IdentityKeySwitch<iksP>(out, in, st, gpu): KeySwitchFromTLWE<iksP> of one
+// extracted TLWE (reference include/keyswitch_gpu.cuh:128-187), lvl10 or lvl20.
+template <class iksP>
+void IdentityKeySwitch(TFHEpp::lvl0param::T* const out, const typename iksP::domainP::T* const in,
+                       const hipStream_t st, const int gpuNum)
+{
+    detail::check_gpu(gpuNum);
+    if constexpr (std::is_same_v<iksP, TFHEpp::lvl10param>)
+        CuCheck(cufhe_identity_key_switch_lvl10(out, in, 1, st), "IdentityKeySwitch<lvl10param>");
+    else {
+        static_assert(std::is_same_v<iksP, TFHEpp::lvl20param>, "lvl10param or lvl20param");
+        CuCheck(cufhe_identity_key_switch_lvl20(out, in, 1, st), "IdentityKeySwitch<lvl20param>");
+    }
+}
+
 // TRLWE extras (reference src/cufhe_gates_gpu.cu:134-196).
 inline void gSampleExtractAndKeySwitch(Ctxt<TFHEpp::lvl0param>& out, const cuFHETRLWElvl1& in,
                                        Stream st)
