@@ -26,11 +26,18 @@ typename P::T* dev_alloc(size_t count)
     return static_cast<typename P::T*>(p);
 }
 
+// Copies go on the stream the launchers use (it does not synchronise with
+// the null stream), and the device is synchronised after each, so an input
+// is on the device before the next launch and a result is read after the
+// launches that write it.
+hipStream_t g_st = nullptr;
+
 template <class P>
 typename P::T* upload(const TLWE<P>& c)
 {
     auto* d = dev_alloc<P>(c.size());
-    cufhe::CuCheck(cufhe_memcpy_h2d(d, c.data(), sizeof(c), nullptr), "h2d");
+    cufhe::CuCheck(cufhe_memcpy_h2d(d, c.data(), sizeof(c), g_st), "h2d");
+    cufhe::Synchronize();
     return d;
 }
 
@@ -38,7 +45,8 @@ template <class P>
 TLWE<P> download(const typename P::T* d)
 {
     TLWE<P> c;
-    cufhe::CuCheck(cufhe_memcpy_d2h(c.data(), d, sizeof(c), nullptr), "d2h");
+    cufhe::Synchronize();
+    cufhe::CuCheck(cufhe_memcpy_d2h(c.data(), d, sizeof(c), g_st), "d2h");
     cufhe::Synchronize();
     return c;
 }
@@ -67,6 +75,7 @@ int main()
     cufhe::Stream s;
     s.Create();
     hipStream_t st = s.st();
+    g_st = st;
     std::mt19937 rng(77);
     int bad = 0;
     constexpr uint32_t mu1 = lvl1param::μ;
@@ -213,7 +222,8 @@ int main()
             auto c2 = tlweSymEncrypt<lvl2param>(b ? lvl2param::μ : -lvl2param::μ, lvl2param::α,
                                                 sk.key.lvl2);
             uint64_t* d2 = dev_alloc<lvl2param>(lvl2param::n + 1);
-            cufhe::CuCheck(cufhe_memcpy_h2d(d2, c2.data(), sizeof(c2), nullptr), "h2d");
+            cufhe::CuCheck(cufhe_memcpy_h2d(d2, c2.data(), sizeof(c2), st), "h2d");
+            cufhe::Synchronize();
             cufhe::IdentityKeySwitch<lvl20param>(o, d2, st, 0);
             e += tlweSymDecrypt<lvl0param>(download<lvl0param>(o), sk.key.lvl0) != b;
             total += 5;
