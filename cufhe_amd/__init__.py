@@ -101,6 +101,7 @@ def lib():
         "cufhe_memcpy_d2d": ([vp, vp, sz, vp], i32),
         "cufhe_plan_jobs_per_group": ([vp], i32),
         "cufhe_plan_run_timed": ([vp, vp, ctypes.POINTER(ctypes.c_float)], i32),
+        "cufhe_set_key_switch_split": ([i32], i32),
         "cufhe_gate_lvl1": ([i32, vp, vp, vp, vp, vp], i32),
         "cufhe_gate_lvl02": ([i32, vp, vp, vp, vp, vp], i32),
         "cufhe_gate_lvl0": ([i32, vp, vp, vp, vp, vp], i32),

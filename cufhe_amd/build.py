@@ -54,7 +54,7 @@ def build(jobs=4, verbose=False):
                 print(out)
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs +
-         ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
+         ["-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     # C++ drop-in tests (tests/cpp/*.cpp), linked against the library in-tree
     root = os.path.dirname(HERE)

@@ -1,4 +1,4 @@
-// Batched identity key switching as an exact int8 GEMM (ks_gemm.hip).
+// Batched identity key switching as an exact int8 MFMA product (ks_gemm.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -10,8 +10,11 @@
 
 namespace cufhe_amd {
 
-// Key-switching key as four signed byte planes: [row][4 (n0 + 1)] int8,
-// row = (switched coefficient, digit, |digit| - 1).
+// Key-switching key pre-tiled for the int8 MFMA key switch: for each block
+// of 32 output words, each 32-deep step of the digit dimension and each of
+// the four balanced byte planes, one 1 KiB operand fragment in register
+// order. rows = switched coefficients x t x 2^(basebit-1) (the digit rows
+// of the TFHEpp layout), cols = 4 (n0 + 1).
 struct KsGemmKey {
     int8_t* planes = nullptr;
     int rows = 0, cols = 0;
@@ -19,13 +22,19 @@ struct KsGemmKey {
     bool subset = false;
 };
 
-// Batches below this many jobs use the gather kernel (k_ks_batch). The GEMM
-// wins at every batch size measured (16 jobs: 0.08 ms against 0.55 ms for
-// the eight-waves-per-job gather), so every plan with a workspace uses it;
-// the gather form serves the entry points that take no workspace.
+// Batches of at least this many jobs use the MFMA key switch when a
+// workspace is given (it wins at every batch size); the gather kernel
+// (k_ks_batch) serves the entry points that take no workspace.
 constexpr int kKsGemmMinJobs = 1;
 
-// Builds the byte planes of a device KSK (TFHEpp layout); 0 on success.
+// Blocks of 32 jobs per wave in the MFMA kernel (MM x 4 accumulators of
+// 32 x 32 int32) and waves per workgroup (sharing one LDS key chunk).
+constexpr int kKsMfmaJobBlocks = 2;
+constexpr int kKsMfmaWaves = 8;
+// Steps of the digit dimension per LDS chunk (one workgroup barrier each).
+constexpr int kKsMfmaChunk = 4;
+
+// Builds the pre-tiled key from a device KSK (TFHEpp layout); 0 on success.
 int ks_gemm_prepare(KsGemmKey& key, const uint32_t* d_ksk, int domain_level, bool subset,
                     hipStream_t st);
 void ks_gemm_release(KsGemmKey& key);
@@ -33,5 +42,9 @@ void ks_gemm_release(KsGemmKey& key);
 size_t ks_gemm_workspace_bytes(const KsGemmKey& key, int njobs);
 // Key-switches njobs jobs (device KsJob array) on st; 0 on success.
 int ks_gemm_run(const KsGemmKey& key, const KsJob* d_jobs, int njobs, void* ws, hipStream_t st);
+// Number of waves the digit dimension is split over (their partial sums are
+// added atomically); 0 restores the per-batch choice (about two waves per
+// SIMD).
+void ks_gemm_set_split(int split);
 
 }  // namespace cufhe_amd

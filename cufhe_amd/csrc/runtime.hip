@@ -1333,6 +1333,13 @@ int cufhe_gate_batch_lvl1(const cufhe_gate_desc* gates, int n, uint32_t* d_pool,
     return rc;
 }
 
+int cufhe_set_key_switch_split(int split)
+{
+    if (split < 0 || split > 64) return fail(CUFHE_ERR_ARG, "key switch split out of range 0-64");
+    ks_gemm_set_split(split);
+    return CUFHE_OK;
+}
+
 int cufhe_identity_key_switch_lvl10(uint32_t* d_out, const uint32_t* d_in, int n, void* stream)
 {
     State& s = state();

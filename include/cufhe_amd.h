@@ -181,6 +181,12 @@ int cufhe_plan_jobs_per_group(const cufhe_plan* plan);
  * wait for it, and return the kernel durations in ms:
  * kernel_ms[0] = key switching, [1] = blind rotation, [2] = combine. */
 int cufhe_plan_run_timed(cufhe_plan* plan, void* stream, float* kernel_ms);
+/* Tuning knob of the batched (int8 MFMA) key switch, no reference
+ * counterpart: the number of waves the digit dimension of each output tile
+ * is split over, their partial sums added atomically (1-64); 0 restores the
+ * per-batch choice (about two waves per SIMD). Applies to launches made
+ * after the call, and to graphs captured after it. */
+int cufhe_set_key_switch_split(int split);
 /* Convenience: plan + run + destroy (allocates per call). */
 int cufhe_gate_batch_lvl1(const cufhe_gate_desc* gates, int n, uint32_t* d_pool,
                           int jobs_per_group, void* stream);

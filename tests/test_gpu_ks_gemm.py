@@ -1,7 +1,10 @@
-"""The batched key switch as an int8 GEMM (every batched plan) is
+"""The batched key switch as an int8 MFMA product (every batched plan) is
 bit-identical to the gather form used by the plain per-gate kernels: full
 lvl10 KSK (FFT key bundle), subset KSK (NTT block-binary) and lvl20 (lvl02
-gates), eagerly and as HIP-graph replays interleaved with eager work."""
+gates), eagerly and as HIP-graph replays interleaved with eager work, with
+the digit dimension unsplit (plain read-add-write epilogue) and split over
+waves (atomic epilogue), at job counts that are not multiples of the
+32-job MFMA block."""
 import numpy as np
 import pytest
 
@@ -87,3 +90,23 @@ def test_lvl20_gemm_key_switch_equals_gather(engine, keys_lvl2):
     outs = _batched(ops, pool_ct, N, W0, 2) + _batched(ops, pool_ct, N, W0, 2, graph=True, replays=2)
     for out in outs:
         assert np.array_equal(out, plain)
+
+
+@pytest.mark.parametrize("split", [1, 5])
+def test_lvl1_mfma_key_switch_split_and_ragged(engine, keys_lvl1, split):
+    """333 gates (10 full 32-job blocks and a partial one) with the split
+    forced to 1 and to 5; the default choice restored afterwards."""
+    load_lvl1_keys(("fft", True), keys_lvl1["bk_kb"], keys_lvl1["ksk"],
+                   backend=C.BACKEND_FFT, variant=C.BR_KEY_BUNDLE)
+    n = 333
+    rng = np.random.default_rng(94 + split)
+    ops = _gates(rng, n)
+    bits = rng.integers(0, 2, 3 * n).astype(np.uint8)
+    pool_ct = C.encrypt_bits(1, 95, keys_lvl1["k1"], bits)
+    plain = _plain(ops, pool_ct, n, W1, C.lib().cufhe_gate_lvl1)
+    C.check(C.lib().cufhe_set_key_switch_split(split), "set split")
+    try:
+        out = _batched(ops, pool_ct, n, W1, 1)[0]
+    finally:
+        C.check(C.lib().cufhe_set_key_switch_split(0), "set split")
+    assert np.array_equal(out, plain)
