@@ -23,6 +23,7 @@ import argparse
 import json
 import os
 import statistics
+import subprocess
 import sys
 import time
 
@@ -179,6 +180,61 @@ def measured_traffic(workload):
         return json.load(f).get(workload, {}).get("hbm_bytes_per_launch")
 
 
+def host_cpu():
+    """Host CPU model and logical CPU count (lscpu, os.cpu_count)."""
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL,
+                             text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(args, C, ops, pool, B, bk, ksk):
+    """lvl1 KB gates of the same mix on the host cores: the project's own CPU
+    engine with the key precomputed in the Fourier domain (oracle/
+    cpu_baseline.cpp), one gate per thread; single-thread latency first."""
+    import numpy as np
+
+    import oracle as O
+    model, ncpu = host_cpu()
+    # one GPU's share of the host: 16 threads, fewer if the host has fewer
+    nthr = args.cpu_threads or min(16, ncpu or 1)
+    eng = O.CpuBaseline(bk, True)
+    try:
+        opc = np.array([C.GATES[o] for o in ops], np.int32)
+        lat = []
+        for i in range(3):
+            t0 = time.perf_counter()
+            eng.gates(opc[i:i + 1], pool[i:i + 1], pool[B + i:B + i + 1], ksk, 1)
+            lat.append((time.perf_counter() - t0) * 1e3)
+        done = 0
+        c0 = time.perf_counter()
+        while time.perf_counter() - c0 < args.cpu_seconds and done < B:
+            m = min(nthr, B - done)
+            eng.gates(opc[done:done + m], pool[done:done + m], pool[B + done:B + done + m], ksk,
+                      nthr)
+            done += m
+        cpu_s = time.perf_counter() - c0
+    finally:
+        eng.release()
+    cpu = {"value": round(done / cpu_s, 3), "unit": "gates/s", "cores": nthr, "kind": "port",
+           "host_cpu_model": model, "host_logical_cpus": ncpu,
+           "single_thread_latency_ms": round(statistics.median(lat), 2),
+           "sample": "%d gates of the same mix in %.1f s; own C++ CPU engine (bootstrapping key "
+                     "precomputed in the Fourier domain, 6 FFNTs per key-bundle step, row-wise "
+                     "integer key switch, AVX2/FMA), one gate per thread; not TFHEpp" %
+                     (done, cpu_s)}
+    log("cpu baseline: %.2f gates/s on %d threads (%s, %s logical CPUs); single-thread latency "
+        "%.1f ms" % (cpu["value"], nthr, model, ncpu, cpu["single_thread_latency_ms"]))
+    return cpu
+
+
 def base_result(args, value, ms_per_step, dtype, config):
     return {
         "metric": METRIC, "value": round(value, 2), "unit": "gates/s", "n_gpus": 1,
@@ -250,23 +306,7 @@ def workload_kb4096(args):
     key_gbps = BK_BYTES_KB * groups / br_avg_s / 1e9
     traffic = measured_traffic("kb4096")
 
-    cpu = None
-    if not args.no_cpu:
-        import oracle as O
-        nthr = args.cpu_threads or min(16, os.cpu_count() or 1)
-        done = 0
-        c0 = time.perf_counter()
-        while time.perf_counter() - c0 < args.cpu_seconds and done < B:
-            m = min(nthr, B - done)
-            opcodes = np.array([C.GATES[o] for o in ops[done:done + m]], np.int32)
-            O.gate_batch_lvl1(O.FFT, True, opcodes, pool[done:done + m],
-                              pool[B + done:B + done + m], bk, ksk, nthr)
-            done += m
-        cpu_s = time.perf_counter() - c0
-        cpu = {"value": round(done / cpu_s, 3), "unit": "gates/s", "cores": nthr, "kind": "port",
-               "sample": "%d gates of the same mix, own C++ CPU restatement (FP64 FFNT, integer "
-                         "key switch), one gate per thread; not TFHEpp" % done}
-        log("cpu baseline: %.2f gates/s on %d threads" % (cpu["value"], nthr))
+    cpu = None if args.no_cpu else cpu_baseline(args, C, ops, pool, B, bk, ksk)
 
     res = base_result(args, value, ms_per_step, "f64", {
         "workload": "batched lvl1 binary gates, KB=2, FFT, 4096 gates uniform over "

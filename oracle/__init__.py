@@ -63,6 +63,10 @@ def lib():
         "oracle_gate_batch_lvl1_ex": ([ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp, vp, vp,
                                        ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "oracle_set_mutation": ([ctypes.c_int], ctypes.c_int),
+        "oracle_cpu_baseline_prepare": ([vp, ctypes.c_int], vp),
+        "oracle_cpu_baseline_release": ([vp], None),
+        "oracle_cpu_baseline_gates": ([vp, vp, vp, vp, vp, vp, ctypes.c_int, ctypes.c_int],
+                                      ctypes.c_int),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -192,6 +196,33 @@ def gate_batch_lvl1(mode, kb, ops, in0, in1, bk, ksk, nthreads, inc=None):
     _ok(lib().oracle_gate_batch_lvl1_ex(mode, int(kb), _p(ops), _p(in0), _p(in1), _p(inc), _p(bk),
                                         _p(ksk), _p(out), len(ops), nthreads), "gate_batch_lvl1")
     return out
+
+
+class CpuBaseline:
+    """The benchmark's CPU-baseline engine (cpu_baseline.cpp): lvl1 binary
+    gates with the bootstrapping key kept in the Fourier domain, one gate per
+    host thread. Not TFHEpp; this project's own CPU implementation."""
+
+    def __init__(self, bk, key_bundle):
+        self._bk = np.ascontiguousarray(bk, np.uint32)
+        self.handle = lib().oracle_cpu_baseline_prepare(_p(self._bk), 1 if key_bundle else 0)
+        if not self.handle:
+            raise RuntimeError("oracle_cpu_baseline_prepare failed")
+
+    def gates(self, ops, in0, in1, ksk, nthreads):
+        ops = np.ascontiguousarray(ops, np.int32)
+        in0 = np.ascontiguousarray(in0, np.uint32)
+        in1 = np.ascontiguousarray(in1, np.uint32)
+        ksk = np.ascontiguousarray(ksk, np.uint32)
+        out = np.zeros_like(in0)
+        _ok(lib().oracle_cpu_baseline_gates(self.handle, _p(ops), _p(in0), _p(in1), _p(ksk),
+                                            _p(out), len(ops), nthreads), "cpu_baseline_gates")
+        return out
+
+    def release(self):
+        if self.handle:
+            lib().oracle_cpu_baseline_release(self.handle)
+            self.handle = None
 
 
 MUTATIONS = {
