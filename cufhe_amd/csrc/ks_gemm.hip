@@ -431,6 +431,10 @@ int run_shape(const KsGemmKey& key, const KsJob* d_jobs, int njobs, void* ws, hi
 
 }  // namespace
 
+static_assert(KsShape<TFHEpp::lvl10param, false>::steps == kKsLvl10StepsMax &&
+                  KsShape<TFHEpp::lvl10param, true>::steps <= kKsLvl10StepsMax,
+              "lvl1 key switch workspace bound");
+
 void ks_gemm_set_split(int split) { g_split_override = split; }
 
 int ks_gemm_prepare(KsGemmKey& key, const uint32_t* d_ksk, int domain_level, bool subset,

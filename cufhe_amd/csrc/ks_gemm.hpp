@@ -31,6 +31,9 @@ constexpr int kKsGemmMinJobs = 1;
 // 32 x 32 int32) and waves per workgroup (sharing one LDS key chunk).
 constexpr int kKsMfmaJobBlocks = 2;
 constexpr int kKsMfmaWaves = 8;
+// 32-deep steps of the digit dimension of the full lvl10 key, the largest
+// lvl1 key switch (workspace: 4 bytes per step per job).
+constexpr int kKsLvl10StepsMax = 448;
 // Steps of the digit dimension per LDS chunk (one workgroup barrier each).
 constexpr int kKsMfmaChunk = 4;
 

@@ -229,6 +229,52 @@ void free_keys(LevelKeys<P>& k)
     k = LevelKeys<P>{};
 }
 
+// ---- scratch of the entry point that allocates nothing -------------------
+// cufhe_identity_key_switch_lvl10_noalloc keeps its job descriptors and the
+// MFMA key switch's code words in this file-scope array (one copy per GPU,
+// held until the process ends). Host code never uses the array's name as an
+// address: the device address is looked up once per device with
+// hipGetSymbolAddress (at cufhe_initialize, or at the first call on another
+// device) and that pointer is what kernels receive.
+constexpr int kNoAllocMaxJobs = 4096;
+constexpr size_t kNoAllocJobBytes = (kNoAllocMaxJobs * sizeof(KsJob) + 255) & ~size_t(255);
+constexpr size_t kNoAllocScratchBytes =
+    kNoAllocJobBytes + static_cast<size_t>(kKsLvl10StepsMax) * kNoAllocMaxJobs * 4;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up once).
+int scratch_on_current_device(void** out)
+{
+    int dev = 0;
+    CUFHE_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(CUFHE_ERR_NO_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(CUFHE_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return CUFHE_OK;
+}
+
+// Job descriptors of n extracted lvl1 TLWEs key-switched to lvl0.
+__global__ __launch_bounds__(256) void k_fill_ks_jobs(KsJob* __restrict__ jobs,
+                                                       const uint32_t* __restrict__ in,
+                                                       uint32_t* __restrict__ out, int n)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+        jobs[i] = KsJob{in + static_cast<size_t>(i) * (lvl1param::n + 1), nullptr,
+                        out + static_cast<size_t>(i) * (lvl0param::n + 1), 1, 0, 0};
+}
+
 int ensure_init()
 {
     State& s = state();
@@ -237,6 +283,11 @@ int ensure_init()
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
         return fail(CUFHE_ERR_NO_DEVICE, "no HIP device visible");
     CUFHE_HIP_TRY(hipSetDevice(s.device));
+    {
+        void* sp = nullptr;
+        const int src = scratch_on_current_device(&sp);
+        if (src != CUFHE_OK) return src;
+    }
     int rc = build_tables(s.t1);
     if (rc != CUFHE_OK) return rc;
     rc = build_tables(s.t2);
@@ -1358,6 +1409,31 @@ int cufhe_identity_key_switch_lvl10(uint32_t* d_out, const uint32_t* d_in, int n
     CUFHE_HIP_TRY(hipFreeAsync(d, st));
     CUFHE_HIP_TRY(hipStreamSynchronize(st));
     return check_launch("identity key switch");
+}
+
+int cufhe_identity_key_switch_lvl10_noalloc(uint32_t* d_out, const uint32_t* d_in, int n,
+                                            void* stream)
+{
+    State& s = state();
+    if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded");
+    if (n < 0) return fail(CUFHE_ERR_ARG, "negative count");
+    if (n > kNoAllocMaxJobs)
+        return fail(CUFHE_ERR_ARG, "identity key switch without allocation takes at most " +
+                                       std::to_string(kNoAllocMaxJobs) + " TLWEs per call");
+    if (n == 0) return CUFHE_OK;
+    if (!d_out || !d_in) return fail(CUFHE_ERR_ARG, "null pointer");
+    if (!s.k1.ksg.planes) return fail(CUFHE_ERR_STATE, "key switch fragments not prepared");
+    void* base = nullptr;
+    const int src = scratch_on_current_device(&base);
+    if (src != CUFHE_OK) return src;
+    KsJob* jobs = static_cast<KsJob*>(base);
+    void* ws = static_cast<unsigned char*>(base) + kNoAllocJobBytes;
+    if (ks_gemm_workspace_bytes(s.k1.ksg, n) > kNoAllocScratchBytes - kNoAllocJobBytes)
+        return fail(CUFHE_ERR_STATE, "no-alloc scratch smaller than the key switch workspace");
+    const hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(k_fill_ks_jobs, dim3((n + 255) / 256), dim3(256), 0, st, jobs, d_in, d_out, n);
+    if (ks_gemm_run(s.k1.ksg, jobs, n, ws, st) != 0) return fail(CUFHE_ERR_HIP, "key switch launch failed");
+    return check_launch("identity key switch (no allocation)");
 }
 
 int cufhe_identity_key_switch_lvl20(uint32_t* d_out, const uint64_t* d_in, int n, void* stream)

@@ -241,6 +241,18 @@ int cufhe_plan_create_lvl01(const cufhe_gate_desc* gates, int n, uint32_t* d_poo
  * include/keyswitch_gpu.cuh:128-187 KeySwitchFromTLWE<lvl10param>).
  * in: n x 1025, out: n x 637 words. */
 int cufhe_identity_key_switch_lvl10(uint32_t* d_out, const uint32_t* d_in, int n, void* stream);
+/* The same key switch for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); it has no counterpart in the reference. It calls no
+ * hipMalloc/hipFree and no allocating library routine: the job descriptors
+ * and the int8 MFMA key switch's digit codes live in a fixed device array of
+ * the library (7.5 MB per GPU, held until the process ends), so n is at most
+ * 4096 (larger n returns CUFHE_ERR_ARG without launching). Asynchronous on
+ * `stream`; the output is bit-identical to cufhe_identity_key_switch_lvl10.
+ * Calls that overlap on one GPU (concurrent streams or threads) share that
+ * array and race: serialise them. */
+int cufhe_identity_key_switch_lvl10_noalloc(uint32_t* d_out, const uint32_t* d_in, int n,
+                                            void* stream);
 /* IdentityKeySwitch lvl2 -> lvl0 with the lvl02 keys (reference
  * include/keyswitch_gpu.cuh:128-187 KeySwitchFromTLWE<lvl20param>, the body
  * rounded from Torus64 to Torus32 as KeySwitchTorusConvert :33-50; its GPU
