@@ -37,6 +37,11 @@ BK_BYTES_KB = 62521344                # 318 x 3 TRGSW x 8 polys x 512 bins x 16 
 BK_BYTES_NTT_BB = 20840448            # 636 TRGSW x 8 polys x 1024 bins x 4 B
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBPS = 8000.0
+# one 64-lane VALU instruction per CU-cycle: 256 CUs x 2.4 GHz
+VALU_PEAK_GINST = 256 * 2.4
+# VALU wave-instructions of one 4096-gate k_br_ntt<2, 2> launch (rocprofv3
+# SQ_INSTS_VALU, profiles/br_ntt_bb_lvl1_batch4096_counters.md)
+NTT_BB_VALU_INSTS_C2 = 7.609e9
 GATE_MIX = ["NAND", "AND", "OR", "XOR", "XNOR", "NOR"]
 
 
@@ -329,7 +334,9 @@ def workload_kb4096(args):
                            "output_noise": vnoise},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": traffic},
+                     "traffic": traffic,
+                     "basis": "algorithmic FP64 flops (FP64 VALU FMAs: the peak is MI355X's FP64 "
+                              "figure, the same for vector and matrix)"},
         "key_stream": {"algorithmic_GBps": round(key_gbps, 1), "peak_GBps": HBM_PEAK_GBPS},
         "cpu_baseline": cpu,
     })
@@ -476,6 +483,7 @@ def workload_nttbb(args):
     br_avg_s = statistics.mean(br_ms) / 1e3
     groups = -(-nbr // best_c)
     key_gbps = BK_BYTES_NTT_BB * groups / br_avg_s / 1e9
+    valu_g = (NTT_BB_VALU_INSTS_C2 / br_avg_s / 1e9) if best_c == 2 else float("nan")
     res = base_result(args, value, total * 1e3 / args.steps, "u32 mod P", {
         "workload": "batched lvl1 NAND/XOR gates, NTT backend (P=2147473409), block-binary "
                     "lvl0 key (blocks of 4), subset key switching, 4096 gates",
@@ -496,9 +504,14 @@ def workload_nttbb(args):
         "verified_chain": {"gates": verified, "layers": vlayers, "decrypt_failures": vfail,
                            "output_noise": vnoise},
         "bit_exact_sample": bit_exact,
-        "roofline": {"bound": "hbm", "achieved": round(key_gbps, 1), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(key_gbps / HBM_PEAK_GBPS, 4),
-                     "traffic": measured_traffic("nttbb")},
+        # integer-only kernel bound by VALU issue: VALU wave-instructions per
+        # launch (profiles/br_ntt_bb_lvl1_batch4096_counters.md, C=2) over
+        # the live launch time, against one wave-instruction per CU-cycle
+        "roofline": {"bound": "valu", "achieved": round(valu_g, 1), "peak": VALU_PEAK_GINST,
+                     "unit": "G wave-instr/s", "frac": round(valu_g / VALU_PEAK_GINST, 4),
+                     "traffic": measured_traffic("nttbb"),
+                     "basis": "measured SQ_INSTS_VALU per launch (no FP64; integer issue bound)"},
+        "key_stream": {"algorithmic_GBps": round(key_gbps, 1), "peak_GBps": HBM_PEAK_GBPS},
         "cpu_baseline": None,
     })
     return res
