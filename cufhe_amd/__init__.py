@@ -124,6 +124,12 @@ def lib():
         "cufhe_ffnt_inverse": ([i32, vp, vp, i32, vp], i32),
         "cufhe_trgsw_to_fourier": ([i32, vp, vp, i32, vp], i32),
         "cufhe_cmux_accumulate": ([i32, vp, vp, u32, vp], i32),
+        "cufhe_trgsw_transform_bytes_lvl1": ([], sz),
+        "cufhe_trgsw_to_transform_lvl1": ([vp, vp, i32, vp], i32),
+        "cufhe_cmux_lvl1": ([vp, vp, vp, vp, i32, vp], i32),
+        "cufhe_cmux_batch_lvl1": ([vp, i32, vp], i32),
+        "tfhe_trgsw_encrypt": ([u64, u64, vp, i32, vp], i32),
+        "tfhe_trlwe_encrypt": ([u64, u64, vp, vp, ctypes.c_double, vp], i32),
         "cufhe_server_create": ([sz, i32, i32, ctypes.POINTER(vp)], i32),
         "cufhe_server_submit": ([vp, vp, vp, vp, ctypes.POINTER(u64)], i32),
         "cufhe_server_wait": ([vp, u64, i32], i32),
@@ -212,6 +218,28 @@ def encrypt_bits(level, seed, key, bits):
     out = np.zeros((len(bits), tlwe_words(level)), dt)
     check(lib().tfhe_tlwe_encrypt_bits(level, seed, _ptr(key), _ptr(bits), len(bits), _ptr(out)),
           "tfhe_tlwe_encrypt_bits")
+    return out
+
+
+def trgsw_encrypt(seed, key1, m, count=1):
+    """count lvl1 TRGSW encryptions of the integers m (array or scalar), each
+    4 x 2 x 1024 words, drawn from the streams (seed, 0..count-1)."""
+    ms = np.broadcast_to(np.asarray(m, np.int64), (count,))
+    out = np.zeros((count, 4 * 2 * N1), np.uint32)
+    for i in range(count):
+        check(lib().tfhe_trgsw_encrypt(seed, i, _ptr(key1), int(ms[i]), _ptr(out[i])),
+              "tfhe_trgsw_encrypt")
+    return out
+
+
+def trlwe_encrypt(seed, key1, polys, alpha=3.42338787018369e-08):
+    """lvl1 TRLWE encryptions (a then b, 2 x 1024 words) of the Torus32
+    polynomials in the rows of polys."""
+    polys = np.ascontiguousarray(np.atleast_2d(polys), np.uint32)
+    out = np.zeros((polys.shape[0], 2 * N1), np.uint32)
+    for i in range(polys.shape[0]):
+        check(lib().tfhe_trlwe_encrypt(seed, i, _ptr(key1), _ptr(polys[i]), alpha, _ptr(out[i])),
+              "tfhe_trlwe_encrypt")
     return out
 
 

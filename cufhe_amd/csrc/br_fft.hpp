@@ -165,20 +165,25 @@ struct BrShape {
 // arithmetic, for waves of the workgroup that take no part in a rotation
 // but must keep step with it (every barrier counts all waves that have not
 // ended); none of the pointer arguments is dereferenced then.
-template <class P, bool KB, bool GHOST = false>
+// DIRECT (with KB false): acc_j <- ExtProd(acc, trgsw), the external
+// product of the accumulator itself by one TRGSW, for CMUX (reference
+// src/bootstrap_gpu.cu:700-813 __CMUXNTT__ computes c0 + ExtProd(c1 - c0, C)).
+template <class P, bool KB, bool GHOST = false, bool DIRECT = false>
 __device__ __forceinline__ void br_step_fft(AccStore<P>& acc, int j, uint32_t a0, uint32_t a1,
                                             const cplx* trgsw0, cplx* my_spec,
                                             const cplx* partner_spec, const FourierKeys<P>& K)
 {
+    static_assert(!(KB && DIRECT), "DIRECT takes a single TRGSW");
+    constexpr bool kSelf = KB || DIRECT;  // digits of acc itself; result assigned
     using T = typename P::T;
     using S = BrShape<P>;
     constexpr int N = S::N, H = S::H, R = S::R, L = S::L, NB = S::NB;
     constexpr size_t TS = kTrgswFourier<P>;
     const int lane = threadIdx.x & 63;
 
-    // Digit source: acc itself (KB) or (X^a0 - 1) acc (plain CMUX).
-    T src[(KB || AccStore<P>::kLds) ? 1 : 2 * R];
-    if constexpr (!GHOST && !KB && !AccStore<P>::kLds) {
+    // Digit source: acc itself (KB, DIRECT) or (X^a0 - 1) acc (plain CMUX).
+    T src[(kSelf || AccStore<P>::kLds) ? 1 : 2 * R];
+    if constexpr (!GHOST && !kSelf && !AccStore<P>::kLds) {
         T* rot = reinterpret_cast<T*>(my_spec);
 #pragma unroll
         for (int r = 0; r < 2 * R; r++) rot[AccStore<P>::coef_index(r, lane)] = acc.reg[r];
@@ -191,7 +196,7 @@ __device__ __forceinline__ void br_step_fft(AccStore<P>& acc, int j, uint32_t a0
         wave_lds_sync();
     }
     auto src_at = [&](int r) -> T {
-        if constexpr (KB) {
+        if constexpr (kSelf) {
             return acc.get(r, lane);
         }
         else if constexpr (AccStore<P>::kLds) {
@@ -305,7 +310,7 @@ __device__ __forceinline__ void br_step_fft(AccStore<P>& acc, int j, uint32_t a0
         const cplx v = accj[r];
         const T lo = double_to_torus<T>(v.x * scale);
         const T hi = double_to_torus<T>(v.y * scale);
-        if constexpr (KB) {
+        if constexpr (kSelf) {
             acc.set(r, lane, lo);
             acc.set(R + r, lane, hi);
         }

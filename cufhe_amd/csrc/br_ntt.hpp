@@ -18,7 +18,7 @@
 
 namespace cufhe_amd {
 
-enum NttMode { NTT_PLAIN = 0, NTT_KB = 1, NTT_BB = 2 };
+enum NttMode { NTT_PLAIN = 0, NTT_KB = 1, NTT_BB = 2, NTT_DIRECT = 3 };
 
 // Block-binary lvl0 key: blocks of ell bits, at most one bit set per block.
 constexpr int kBlockEll = TFHEpp::lvl0bbparam::ell;
@@ -71,6 +71,8 @@ __device__ __forceinline__ int ntt_coef_index(int r, int lane) { return lane + 6
 //   NTT_PLAIN: acc += ExtProd((X^a[0] - 1) acc, BK)            (trgsw0 = BK_i)
 //   NTT_KB   : acc  = ExtProd(acc, one + BK2 X(a1) + BK1 X(a0) + BK0 X(a0+a1))
 //   NTT_BB   : acc += ExtProd(acc, sum_l BK_l X(a_l)), l < ell  (trgsw0 = BK_block*ell)
+//   NTT_DIRECT: acc = ExtProd(acc, C)  (one TRGSW, for CMUX; reference
+//              src/bootstrap_gpu.cu:925-1041 __CMUXNTT__)
 // GHOST: the same loop nest and workgroup barriers with no loads, stores or
 // arithmetic (no pointer argument is dereferenced), for the waves of a
 // workgroup that do not rotate but must keep step with the ones that do.
@@ -81,7 +83,7 @@ __device__ __forceinline__ void br_step_ntt(uint32_t (&acc)[16], int j, const ui
 {
     constexpr int N = lvl1param::n, L = lvl1param::l;
     constexpr size_t TS = kTrgswNtt;
-    constexpr int NK = MODE == NTT_PLAIN ? 1 : (MODE == NTT_KB ? 3 : kBlockEll);
+    constexpr int NK = (MODE == NTT_PLAIN || MODE == NTT_DIRECT) ? 1 : (MODE == NTT_KB ? 3 : kBlockEll);
     const int lane = threadIdx.x & 63;
 
     uint32_t src[16];
@@ -106,7 +108,7 @@ __device__ __forceinline__ void br_step_ntt(uint32_t (&acc)[16], int j, const ui
         for (int r = 0; r < 16; r++) src[r] = acc[r];
     }
 
-    XaiNtt xg[MODE == NTT_PLAIN ? 1 : NK];
+    XaiNtt xg[(MODE == NTT_PLAIN || MODE == NTT_DIRECT) ? 1 : NK];
     if constexpr (GHOST) {
     }
     else if constexpr (MODE == NTT_KB) {
@@ -126,7 +128,7 @@ __device__ __forceinline__ void br_step_ntt(uint32_t (&acc)[16], int j, const ui
 
 #pragma unroll 1
     for (int q = 0; q < L; q++) {
-        if constexpr (MODE != NTT_PLAIN && !GHOST) {
+        if constexpr (MODE != NTT_PLAIN && MODE != NTT_DIRECT && !GHOST) {
 #pragma unroll
             for (int e = 0; e < NK; e++) xg[e].pin();
         }
@@ -166,7 +168,7 @@ __device__ __forceinline__ void br_step_ntt(uint32_t (&acc)[16], int j, const ui
             for (int comp = 0; comp < 2; comp++) {
                 D[comp] = (comp == j ? my_lds : partner_lds)[d * 64 + lane];
                 uint32_t k;
-                if constexpr (MODE == NTT_PLAIN) {
+                if constexpr (MODE == NTT_PLAIN || MODE == NTT_DIRECT) {
                     k = kv[sl][comp][0];
                 }
                 else if constexpr (MODE == NTT_KB) {
@@ -193,7 +195,7 @@ __device__ __forceinline__ void br_step_ntt(uint32_t (&acc)[16], int j, const ui
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         const uint32_t t = ntt31_to_torus32(s[r]);
-        acc[r] = MODE == NTT_KB ? t : acc[r] + t;
+        acc[r] = (MODE == NTT_KB || MODE == NTT_DIRECT) ? t : acc[r] + t;
     }
 }
 

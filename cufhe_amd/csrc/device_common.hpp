@@ -137,6 +137,33 @@ struct KsJob {
 };
 static_assert(sizeof(KsJob) == 40, "KsJob layout");
 
+// One lvl1 CMUX res = c0 + ExtProd(c1 - c0, cs): TRLWEs of 2 x N words, cs a
+// TRGSW in the loaded backend's transform domain. Same layout as the C ABI's
+// cufhe_cmux_desc.
+struct CmuxDesc {
+    uint32_t* res;
+    const void* cs;
+    const uint32_t* c1;
+    const uint32_t* c0;
+};
+static_assert(sizeof(CmuxDesc) == 32, "CmuxDesc layout");
+// A batch of CMUXes: per-CMUX descriptors (device array), or, when descs is
+// null, CMUX i at res/c1/c0 + 2N i and cs + cs_bytes i.
+struct CmuxBatch {
+    const CmuxDesc* descs;
+    uint32_t* res;
+    const void* cs;
+    const uint32_t* c1;
+    const uint32_t* c0;
+    size_t cs_bytes;
+    __device__ __forceinline__ CmuxDesc at(int i) const
+    {
+        if (descs) return descs[i];
+        const size_t o = static_cast<size_t>(i) * 2 * lvl1param::n;
+        return CmuxDesc{res + o, static_cast<const char*>(cs) + cs_bytes * i, c1 + o, c0 + o};
+    }
+};
+
 // Elementwise combination of TLWEs (MUX/NMUX combine, NOT, COPY):
 // out[i] = s1*r1[i] + s0*r0[i], plus offset on the body word (i = words-1).
 struct LinJob {

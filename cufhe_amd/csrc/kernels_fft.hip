@@ -685,8 +685,44 @@ __global__ __launch_bounds__(128, 2) void k_cmux_fft(typename P::T* trlwe, const
         trlwe[wave * N + AccStore<P>::coef_index(r, lane)] = acc.get(r, lane);
 }
 
+// Batched lvl1 CMUX (reference src/bootstrap_gpu.cu:700-813 __CMUXNTT__,
+// FFT build): res = c0 + ExtProd(c1 - c0, C), one workgroup of two waves per
+// CMUX (wave j owns component j; the pair exchanges digit spectra through
+// LDS). Each wave reads its own component of c1 and c0 before the first
+// barrier and writes its component of res last, so res may alias c0 or c1.
+template <class P>
+__global__ __launch_bounds__(128, 2) void k_cmux_batch_fft(CmuxBatch batch, FourierKeys<P> K)
+{
+    static_assert(!AccStore<P>::kLds, "register accumulator (lvl1)");
+    constexpr int N = P::n, R = N / 128;
+    constexpr int NB = BrShape<P>::NB;
+    __shared__ cplx sh_spec[2][NB * (N / 2)];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const CmuxDesc d = batch.at(blockIdx.x);
+    AccStore<P> acc;
+#pragma unroll
+    for (int r = 0; r < 2 * R; r++) {
+        const int i = wave * N + AccStore<P>::coef_index(r, lane);
+        acc.set(r, lane, d.c1[i] - d.c0[i]);
+    }
+    br_step_fft<P, false, false, true>(acc, wave, 0, 0, static_cast<const cplx*>(d.cs),
+                                       sh_spec[wave], sh_spec[wave ^ 1], K);
+#pragma unroll
+    for (int r = 0; r < 2 * R; r++) {
+        const int i = wave * N + AccStore<P>::coef_index(r, lane);
+        d.res[i] = d.c0[i] + acc.get(r, lane);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
+
+void launch_cmux_batch_fft(const CmuxBatch& batch, int n, const FourierKeys<lvl1param>& K,
+                           hipStream_t st)
+{
+    hipLaunchKernelGGL((k_cmux_batch_fft<lvl1param>), dim3(n), dim3(128), 0, st, batch, K);
+}
 
 template <class P>
 void launch_to_fourier(cplx* out, const void* in, int npolys, bool integer,

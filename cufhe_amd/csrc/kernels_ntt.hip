@@ -280,8 +280,40 @@ __global__ __launch_bounds__(512, 1) void k_gate_lvl1_ntt(
     }
 }
 
+// Batched lvl1 CMUX on the NTT backend (reference src/bootstrap_gpu.cu:925-1041
+// __CMUXNTT__): res = c0 + ExtProd(c1 - c0, C), one workgroup of two waves
+// per CMUX; res may alias c0 or c1 (each wave reads its component of both
+// before the first barrier and writes it last).
+__global__ __launch_bounds__(128, 2) void k_cmux_batch_ntt(CmuxBatch batch, NttKeys K)
+{
+    constexpr int N = lvl1param::n;
+    __shared__ uint32_t sh_lds[2][kNttLdsWords];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const CmuxDesc d = batch.at(blockIdx.x);
+    uint32_t acc[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int i = wave * N + ntt_coef_index(r, lane);
+        acc[r] = d.c1[i] - d.c0[i];
+    }
+    const uint32_t a[3] = {0u, 0u, 0u};
+    br_step_ntt<NTT_DIRECT>(acc, wave, a, static_cast<const uint32_t*>(d.cs), sh_lds[wave],
+                            sh_lds[wave ^ 1], K);
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int i = wave * N + ntt_coef_index(r, lane);
+        d.res[i] = d.c0[i] + acc[r];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Launchers.
+void launch_cmux_batch_ntt(const CmuxBatch& batch, int n, const NttKeys& K, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_cmux_batch_ntt, dim3(n), dim3(128), 0, st, batch, K);
+}
+
 void launch_to_ntt(uint32_t* out, const uint32_t* in, int npolys, const NttTables& tb,
                    hipStream_t st)
 {

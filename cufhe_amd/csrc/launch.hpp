@@ -52,6 +52,9 @@ void launch_cmux_fft(typename P::T* trlwe, const cplx* trgsw, uint32_t abar,
                      const FourierKeys<P>& K, hipStream_t st);
 
 // NTT backend (lvl1): mode 0 plain CMUX, 1 key bundle, 2 block-binary.
+void launch_cmux_batch_fft(const CmuxBatch& batch, int n, const FourierKeys<lvl1param>& K,
+                           hipStream_t st);
+void launch_cmux_batch_ntt(const CmuxBatch& batch, int n, const NttKeys& K, hipStream_t st);
 void launch_to_ntt(uint32_t* out, const uint32_t* in, int npolys, const NttTables& tb,
                    hipStream_t st);
 void launch_ntt_natural(bool inverse, uint32_t* out, const uint32_t* in, int npolys,

@@ -1562,6 +1562,60 @@ int cufhe_ffnt_inverse(int N, const void* d_spectra, double* d_poly, int count, 
     return check_launch("ffnt inverse");
 }
 
+size_t cufhe_trgsw_transform_bytes_lvl1(void)
+{
+    const LevelKeys<lvl1param>& k = state().k1;
+    if (!k.loaded) return 0;
+    return k.backend == CUFHE_BACKEND_NTT ? kTrgswNtt * sizeof(uint32_t)
+                                          : kTrgswFourier<lvl1param> * sizeof(cplx);
+}
+
+int cufhe_trgsw_to_transform_lvl1(const uint32_t* d_trgsw, void* d_out, int count, void* stream)
+{
+    State& s = state();
+    if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded (they select the backend)");
+    if (count < 0) return fail(CUFHE_ERR_ARG, "negative count");
+    if (count == 0) return CUFHE_OK;
+    if (!d_trgsw || !d_out) return fail(CUFHE_ERR_ARG, "null pointer");
+    const int npolys = count * TFHEpp::BootstrappingTRGSWRows<lvl1param> * 2;
+    const hipStream_t st = as_stream(stream);
+    if (s.k1.backend == CUFHE_BACKEND_NTT)
+        launch_to_ntt(static_cast<uint32_t*>(d_out), d_trgsw, npolys, s.tn.tb, st);
+    else
+        launch_to_fourier<lvl1param>(static_cast<cplx*>(d_out), d_trgsw, npolys, false, s.k1.fk, st);
+    return check_launch("TRGSW transform");
+}
+
+static int cmux_launch(const CmuxBatch& b, int n, void* stream)
+{
+    State& s = state();
+    if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded (they select the backend)");
+    if (n < 0) return fail(CUFHE_ERR_ARG, "negative count");
+    if (n == 0) return CUFHE_OK;
+    const hipStream_t st = as_stream(stream);
+    if (s.k1.backend == CUFHE_BACKEND_NTT)
+        launch_cmux_batch_ntt(b, n, s.k1.nk, st);
+    else
+        launch_cmux_batch_fft(b, n, s.k1.fk, st);
+    return check_launch("CMUX");
+}
+
+int cufhe_cmux_lvl1(uint32_t* d_res, const void* d_cs, const uint32_t* d_c1, const uint32_t* d_c0,
+                    int n, void* stream)
+{
+    if (n > 0 && (!d_res || !d_cs || !d_c1 || !d_c0)) return fail(CUFHE_ERR_ARG, "null pointer");
+    CmuxBatch b{nullptr, d_res, d_cs, d_c1, d_c0, cufhe_trgsw_transform_bytes_lvl1()};
+    return cmux_launch(b, n, stream);
+}
+
+int cufhe_cmux_batch_lvl1(const cufhe_cmux_desc* d_descs, int n, void* stream)
+{
+    static_assert(sizeof(cufhe_cmux_desc) == sizeof(CmuxDesc), "descriptor layout");
+    if (n > 0 && !d_descs) return fail(CUFHE_ERR_ARG, "null pointer");
+    CmuxBatch b{reinterpret_cast<const CmuxDesc*>(d_descs), nullptr, nullptr, nullptr, nullptr, 0};
+    return cmux_launch(b, n, stream);
+}
+
 int cufhe_trgsw_to_fourier(int N, const void* d_trgsw, void* d_spectra, int count, void* stream)
 {
     if (N == 1024)

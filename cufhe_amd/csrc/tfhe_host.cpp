@@ -394,6 +394,14 @@ int tfhe_trlwe_encrypt(uint64_t seed, uint64_t index, const uint32_t* key, const
     return CUFHE_OK;
 }
 
+int tfhe_trgsw_encrypt(uint64_t seed, uint64_t index, const uint32_t* key, int m, uint32_t* ct)
+{
+    if (!key || !ct) return CUFHE_ERR_ARG;
+    ChaCha20 rng(seed, kTagEnc | index);
+    trgsw_encrypt<lvl1param>(ct, m, key, rng);
+    return CUFHE_OK;
+}
+
 int tfhe_tlwe_phase(int level, const void* key, const void* ct, size_t count, double* out)
 {
     if (!key || (!ct && count) || (!out && count)) return CUFHE_ERR_ARG;

@@ -307,6 +307,34 @@ int cufhe_ffnt_inverse(int N, const void* d_spectra, double* d_poly, int count, 
 int cufhe_ntt_transform(const uint32_t* d_in, uint32_t* d_out, int count, int inverse,
                         void* stream);
 uint32_t cufhe_ntt_root(void);
+/* ---- CMUX / TRGSW API (lvl1, the loaded lvl1 backend) ---------------------
+ * TRGSW2NTT (reference src/bootstrap_gpu.cu:165-186, include/cufhe_gpu.cuh:
+ * 219): count lvl1 TRGSWs in the polynomial domain (TFHEpp layout, 4 rows x
+ * 2 polynomials x 1024 words each) to the transform domain of the loaded
+ * lvl1 keys' backend: FFT spectra (65,536 B per TRGSW) or NTT residues
+ * (32,768 B), the bootstrapping key's storage form.
+ * cufhe_trgsw_transform_bytes_lvl1() is that size (0 before keys load). */
+size_t cufhe_trgsw_transform_bytes_lvl1(void);
+int cufhe_trgsw_to_transform_lvl1(const uint32_t* d_trgsw, void* d_out, int count, void* stream);
+/* CMUXNTT (reference src/cufhe_gates_gpu.cu:116-133, launcher
+ * src/bootstrap_gpu.cu:2357-2369, kernels :700-813 FFT and :925-1041 NTT):
+ * res = c0 + ExtProd(c1 - c0, cs), i.e. c1 if cs encrypts 1 and c0 if it
+ * encrypts 0, for n independent CMUXes in one launch: cs is n transformed
+ * TRGSWs, c1 / c0 / res n lvl1 TRLWEs of 2 x 1024 words (a then b). res may
+ * alias c0 or c1. Asynchronous on `stream`. */
+int cufhe_cmux_lvl1(uint32_t* d_res, const void* d_cs, const uint32_t* d_c1, const uint32_t* d_c0,
+                    int n, void* stream);
+/* The same with one device descriptor per CMUX (any pointers, e.g. a CMUX
+ * tree level whose selectors repeat); the descriptor array is in device
+ * memory. */
+typedef struct {
+    uint32_t* res;
+    const void* cs;
+    const uint32_t* c1;
+    const uint32_t* c0;
+} cufhe_cmux_desc;
+int cufhe_cmux_batch_lvl1(const cufhe_cmux_desc* d_descs, int n, void* stream);
+
 /* TRGSW (polynomial domain, TFHEpp layout) -> spectra, count TRGSWs. */
 int cufhe_trgsw_to_fourier(int N, const void* d_trgsw, void* d_spectra, int count,
                            void* stream);

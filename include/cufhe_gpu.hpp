@@ -141,6 +141,31 @@ struct cuFHETRLWElvl1 {
     std::vector<uint32_t*> trlwedevices;
 };
 
+/** lvl1 TRGSW in the transform domain of the loaded lvl1 backend (reference
+ *  cufhe_gpu.cuh:136-156 cuFHETRGSWNTTlvl1): FFT spectra or NTT residues,
+ *  cufhe_trgsw_transform_bytes_lvl1() bytes; construct after the keys are
+ *  loaded (they select the backend). */
+struct cuFHETRGSWNTTlvl1 {
+    cuFHETRGSWNTTlvl1() : bytes(cufhe_trgsw_transform_bytes_lvl1())
+    {
+        if (bytes == 0) {
+            std::fprintf(stderr, "cuFHETRGSWNTTlvl1: load the lvl1 keys first\n");
+            std::exit(-1);
+        }
+        void* p = nullptr;
+        CuCheck(cufhe_device_alloc(&p, bytes), "TRGSW alloc");
+        trgswhost.resize(bytes);
+        trgswdevices.assign(_gpuNum, p);
+    }
+    ~cuFHETRGSWNTTlvl1() { cufhe_device_free(trgswdevices[0]); }
+    cuFHETRGSWNTTlvl1(const cuFHETRGSWNTTlvl1&) = delete;
+    cuFHETRGSWNTTlvl1& operator=(const cuFHETRGSWNTTlvl1&) = delete;
+
+    size_t bytes;
+    std::vector<uint8_t> trgswhost;  // host copy of the transformed TRGSW
+    std::vector<void*> trgswdevices;
+};
+
 template <class P>
 inline void CtxtCopyH2D(Ctxt<P>& c, Stream st)
 {
@@ -450,6 +475,51 @@ void IdentityKeySwitch(TFHEpp::lvl0param::T* const out, const typename iksP::dom
         static_assert(std::is_same_v<iksP, TFHEpp::lvl20param>, "lvl10param or lvl20param");
         CuCheck(cufhe_identity_key_switch_lvl20(out, in, 1, st), "IdentityKeySwitch<lvl20param>");
     }
+}
+
+// TRGSW2NTT(out, trgsw, st) (reference src/bootstrap_gpu.cu:165-186): the
+// TRGSW to the transform domain on the device, and back into out's host copy
+// (asynchronous; valid after Synchronize()).
+inline void TRGSW2NTT(cuFHETRGSWNTTlvl1& out, const TFHEpp::TRGSW<TFHEpp::lvl1param>& trgsw,
+                      Stream& st)
+{
+    void* d = nullptr;
+    CuCheck(cufhe_device_alloc(&d, sizeof(trgsw)), "TRGSW2NTT alloc");
+    CuCheck(cufhe_memcpy_h2d(d, trgsw.data(), sizeof(trgsw), st.st()), "TRGSW2NTT H2D");
+    CuCheck(cufhe_trgsw_to_transform_lvl1(static_cast<const uint32_t*>(d), out.trgswdevices[0], 1,
+                                          st.st()),
+            "TRGSW2NTT");
+    CuCheck(cufhe_memcpy_d2h(out.trgswhost.data(), out.trgswdevices[0], out.bytes, st.st()),
+            "TRGSW2NTT D2H");
+    CuCheck(cufhe_stream_synchronize(st.st()), "TRGSW2NTT sync");
+    cufhe_device_free(d);
+}
+
+// CMUXNTT(res, cs, c1, c0, st) (reference src/cufhe_gates_gpu.cu:116-133):
+// res = c1 if cs encrypts 1, c0 if it encrypts 0. Host copies in and out as
+// the reference does; gCMUXNTT works on the device copies only.
+inline void gCMUXNTT(cuFHETRLWElvl1& res, cuFHETRGSWNTTlvl1& cs, cuFHETRLWElvl1& c1,
+                     cuFHETRLWElvl1& c0, Stream st)
+{
+    CuCheck(cufhe_cmux_lvl1(res.trlwedevices[0], cs.trgswdevices[0], c1.trlwedevices[0],
+                            c0.trlwedevices[0], 1, st.st()),
+            "CMUXNTT");
+}
+inline void CMUXNTT(cuFHETRLWElvl1& res, cuFHETRGSWNTTlvl1& cs, cuFHETRLWElvl1& c1,
+                    cuFHETRLWElvl1& c0, Stream st)
+{
+    CuCheck(cufhe_memcpy_h2d(cs.trgswdevices[0], cs.trgswhost.data(), cs.bytes, st.st()),
+            "CMUXNTT H2D");
+    CuCheck(cufhe_memcpy_h2d(c1.trlwedevices[0], c1.trlwehost.data(), sizeof(c1.trlwehost),
+                             st.st()),
+            "CMUXNTT H2D");
+    CuCheck(cufhe_memcpy_h2d(c0.trlwedevices[0], c0.trlwehost.data(), sizeof(c0.trlwehost),
+                             st.st()),
+            "CMUXNTT H2D");
+    gCMUXNTT(res, cs, c1, c0, st);
+    CuCheck(cufhe_memcpy_d2h(res.trlwehost.data(), res.trlwedevices[0], sizeof(res.trlwehost),
+                             st.st()),
+            "CMUXNTT D2H");
 }
 
 // TRLWE extras (reference src/cufhe_gates_gpu.cu:134-196).

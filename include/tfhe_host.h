@@ -65,6 +65,12 @@ int tfhe_tlwe_encrypt(int level, uint64_t seed, uint64_t index, const void* key,
 int tfhe_trlwe_encrypt(uint64_t seed, uint64_t index, const uint32_t* key_lvl1, const uint32_t* m,
                        double alpha, uint32_t* ct_out);
 
+/* One lvl1 TRGSW encryption of the integer m (TFHEpp trgswSymEncrypt: 4
+ * TRLWE encryptions of zero, m times the gadget added to row r's component
+ * r / l), 4 x 2 x 1024 words in the bootstrapping key's TRGSW layout. */
+int tfhe_trgsw_encrypt(uint64_t seed, uint64_t index, const uint32_t* key_lvl1, int m,
+                       uint32_t* ct_out);
+
 /* Phase of count TLWEs as a torus fraction in [-1/2, 1/2). */
 int tfhe_tlwe_phase(int level, const void* key, const void* ct, size_t count,
                     double* phase_out);

@@ -169,6 +169,19 @@ void trlweSymEncrypt(TRLWE<P>& c, const Polynomial<P>& p, const Key<P>& key)
     trlweSymEncrypt(c, p, key);
 }
 
+// TFHEpp trgswSymEncrypt<lvl1param>(c, m, key): TRGSW of the integer m.
+inline void trgswSymEncrypt(TRGSW<lvl1param>& c, int m, const Key<lvl1param>& key)
+{
+    std::array<uint32_t, kTRGSWWords<lvl1param>> flat;
+    detail::check(tfhe_trgsw_encrypt(detail::encrypt_seed(), detail::encrypt_counter()++,
+                                     key.data(), m, flat.data()),
+                  "tfhe_trgsw_encrypt");
+    for (uint32_t r = 0; r < BootstrappingTRGSWRows<lvl1param>; r++)
+        for (uint32_t j = 0; j < 2; j++)
+            for (uint32_t i = 0; i < lvl1param::n; i++)
+                c[r][j][i] = flat[(static_cast<size_t>(r) * 2 + j) * lvl1param::n + i];
+}
+
 // TFHEpp's in-place form: tlweSymEncrypt<P>(c, p, key) with the level's noise.
 template <class P>
 void tlweSymEncrypt(TLWE<P>& c, const typename P::T p, const Key<P>& key)

@@ -5,7 +5,8 @@
 //   XxxBootstrap<lvl10param, lvl01param, mu>  (IKS-BR, lvl1 in/out)
 //   XxxBootstrap<lvl02param, mu, lvl20param>  (BR-IKS through lvl2)
 //   Mux/NMux/Copy/Not launchers, Bootstrap(out, in, mu), SEIandKS,
-//   BootstrapTLWE2TRLWE, IdentityKeySwitch<lvl10param / lvl20param>.
+//   BootstrapTLWE2TRLWE, IdentityKeySwitch<lvl10param / lvl20param>,
+//   TRGSW2NTT and CMUXNTT.
 // Raw device pointers and hipStream_t, as the reference's launchers take.
 // Prints PASS/FAIL per group and ALL PASS, returns 0/1.
 #include <cstdio>
@@ -234,6 +235,38 @@ int main()
             cufhe_device_free(d2);
         }
         bad += report("Bootstrap / SEIandKS / IdentityKeySwitch", e, total);
+    }
+    // TRGSW2NTT / CMUXNTT (reference src/cufhe_gates_gpu.cu:116-133,
+    // src/bootstrap_gpu.cu:165-186): the result decrypts to c1's message when
+    // the TRGSW encrypts 1 and to c0's when it encrypts 0
+    {
+        int e = 0, total = 0;
+        for (int i = 0; i < 6; i++) {
+            const int b = i & 1;
+            TRGSW<lvl1param> g;
+            trgswSymEncrypt(g, b, sk.key.lvl1);
+            cufhe::cuFHETRGSWNTTlvl1 cs;
+            cufhe::TRGSW2NTT(cs, g, s);
+            cufhe::cuFHETRLWElvl1 c1, c0, res;
+            Polynomial<lvl1param> p1, p0;
+            for (uint32_t k = 0; k < lvl1param::n; k++) {
+                p1[k] = (rng() & 1) ? mu1 : 0u - mu1;
+                p0[k] = (rng() & 1) ? mu1 : 0u - mu1;
+            }
+            trlweSymEncrypt(c1.trlwehost, p1, sk.key.lvl1);
+            trlweSymEncrypt(c0.trlwehost, p0, sk.key.lvl1);
+            cufhe::CMUXNTT(res, cs, c1, c0, s);
+            cufhe::Synchronize();
+            // SampleExtract of coefficient 0 on the host
+            TLWE<lvl1param> t;
+            t[0] = res.trlwehost[0][0];
+            for (uint32_t k = 1; k < lvl1param::n; k++) t[k] = 0u - res.trlwehost[0][lvl1param::n - k];
+            t[lvl1param::n] = res.trlwehost[1][0];
+            const uint32_t want = b ? p1[0] : p0[0];
+            e += tlweSymDecrypt<lvl1param>(t, sk.key.lvl1) != (want == mu1);
+            total++;
+        }
+        bad += report("TRGSW2NTT / CMUXNTT", e, total);
     }
     s.Destroy();
     cufhe::CleanUp();
