@@ -533,21 +533,18 @@ def workload_serving(args):
     C.check(C.lib().cufhe_initialize(), "initialize")
     C.initialize_lvl01(bk, ksk, backend=C.BACKEND_FFT, variant=C.BR_KEY_BUNDLE)
     rng = np.random.default_rng(args.seed)
-    rates = [float(r) for r in args.serve_rates.split(",")]
-    levels = []
-    for li, rate in enumerate(rates):
+
+    def make(n, tag):
         circs, ins, bits = [], [], []
-        for i in range(args.serve_circuits):
+        for i in range(n):
             c = random_circuit(rng, int(rng.integers(64, 1025)), int(rng.integers(4, 33)))
             b = rng.integers(0, 2, c.n_inputs).astype(np.uint8)
             circs.append(c)
             bits.append(b)
-            ins.append(C.encrypt_bits(1, 10000 * (li + 1) + i, k1, b))
-        arrivals = np.cumsum(rng.exponential(1.0 / rate, len(circs)))
-        levels.append((rate, circs, ins, bits, arrivals))
-    log("%d circuits per level, rates %s circuits/s" % (args.serve_circuits, rates))
-    results = []
-    for rate, circs, ins, bits, arrivals in levels:
+            ins.append(C.encrypt_bits(1, 10000 * tag + i, k1, b))
+        return circs, ins, bits
+
+    def serve(circs, ins, arrivals):
         srv = C.Server(pool_bytes=int(args.serve_pool_gb * (1 << 30)),
                        jobs_per_group=args.jobs_per_group)
         ids = [None] * len(circs)
@@ -567,14 +564,35 @@ def workload_serving(args):
         infos = [srv.info(r) for r in ids]
         st = srv.stats()
         srv.destroy()
-        lat = [1e3 * (f["done"] - f["submit"]) for f in infos]
         span = max(f["done"] for f in infos) - min(f["submit"] for f in infos)
+        return outs, infos, st, span
+
+    # capacity: a burst of circuits submitted at once (the server runs at
+    # its full co-batching depth); circuits per second it sustains
+    circs, ins, _ = make(args.serve_burst, 1)
+    _, _, st, span = serve(circs, ins, np.zeros(len(circs)))
+    cap_gates = sum(c.n_gates for c in circs) / span
+    cap = len(circs) / span
+    log("capacity (burst of %d circuits): %.1f circuits/s, %.0f gates/s, GPU busy %.2f"
+        % (len(circs), cap, cap_gates, st["busy_s"] / span))
+    if args.serve_rates:
+        rates = [float(r) for r in args.serve_rates.split(",")]
+    else:
+        rates = [round(f * cap, 2) for f in (float(x) for x in args.serve_levels.split(","))]
+    results = []
+    for li, rate in enumerate(rates):
+        circs, ins, bits = make(args.serve_circuits, li + 2)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, len(circs)))
+        outs, infos, st, span = serve(circs, ins, arrivals)
+        lat = [1e3 * (f["done"] - f["submit"]) for f in infos]
         gates = sum(c.n_gates for c in circs)
         fails = sum(int(np.sum(C.decrypt_bits(1, k1, o) != c.evaluate(b)))
                     for o, c, b in zip(outs, circs, bits))
-        r = {"rate_circuits_per_s": rate, "circuits": len(circs), "gates": gates,
+        r = {"rate_circuits_per_s": rate, "load_vs_capacity": round(rate / cap, 3),
+             "circuits": len(circs), "gates": gates,
              "gates_per_s": round(gates / span, 1),
              "p50_latency_ms": round(pct(lat, 50), 1), "p99_latency_ms": round(pct(lat, 99), 1),
+             "max_latency_ms": round(max(lat), 1),
              "rounds": st["rounds"], "mean_gates_per_round": round(st["gates"] / st["rounds"], 1),
              "gpu_busy_frac": round(st["busy_s"] / span, 3),
              "peak_pool_MiB": round(st["peak_pool_bytes"] / 2**20, 1),
@@ -585,13 +603,15 @@ def workload_serving(args):
     res = base_result(args, top["gates_per_s"], 1e3 / max(top["gates_per_s"], 1e-9), "f64", {
         "workload": "serving: Poisson arrivals of random layered lvl1 circuits (width U[64,1024], "
                     "depth U[4,32], 10%% MUX, binary gates uniform), KB=2 FFT, device-memory "
-                    "admission (%g GiB pool), layers of all admitted circuits co-batched per round"
-                    % args.serve_pool_gb,
+                    "admission (%g GiB pool), layers of all admitted circuits co-batched per round, "
+                    "two rounds in flight" % args.serve_pool_gb,
         "rates_circuits_per_s": rates, "circuits_per_rate": args.serve_circuits,
         "parallelism": "single GPU",
     })
     res["metric"] = "serving gates/s and p50/p99 per-circuit latency (lvl1 circuits, KB=2)"
-    res.update({"levels": results,
+    res.update({"capacity": {"circuits_per_s": round(cap, 2), "gates_per_s": round(cap_gates, 1),
+                             "burst_circuits": args.serve_burst},
+                "levels": results,
                 "p50_latency_ms": top["p50_latency_ms"], "p99_latency_ms": top["p99_latency_ms"],
                 "decrypt_failures": sum(r["decrypt_failures"] for r in results),
                 "roofline": None, "cpu_baseline": None})
@@ -621,8 +641,12 @@ def main():
     ap.add_argument("--verify-gates", type=int, default=1 << 20,
                     help="kb4096, nttbb: gates decrypted and checked after timing, as chained "
                          "layers (0 = off)")
-    ap.add_argument("--serve-rates", default="2,5,10", help="circuits/s, one level each")
-    ap.add_argument("--serve-circuits", type=int, default=24, help="circuits per rate level")
+    ap.add_argument("--serve-rates", default="",
+                    help="circuits/s, one level each (default: --serve-levels of the capacity)")
+    ap.add_argument("--serve-levels", default="0.3,0.6,0.9",
+                    help="arrival rates as fractions of the measured burst capacity")
+    ap.add_argument("--serve-burst", type=int, default=32, help="circuits of the capacity burst")
+    ap.add_argument("--serve-circuits", type=int, default=64, help="circuits per rate level")
     ap.add_argument("--serve-pool-gb", type=float, default=4.0)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the workload, copy this many MiB device-to-device (a kernel of "

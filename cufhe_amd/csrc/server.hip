@@ -6,13 +6,23 @@
 // until its wires fit; at admission its inputs are copied into its range and
 // every later layer's outputs are written next to them, so ciphertexts stay
 // in HBM for the circuit's whole life. Each round the worker thread takes
-// the next layer of every admitted circuit (in admission order, whole layers,
-// up to the round's bootstrap and gate budget), rewrites their local wire ids
-// to pool slots, and runs them as one batch: key switch -> blind rotation ->
-// combine (engine.hpp). Circuits whose last layer ran get their outputs
-// copied back and their range freed, which admits waiting circuits in the
-// next round. Gates never depend on their batch-mates, so every result is
-// bit-identical to running the circuit alone.
+// the next unscheduled layer of every admitted circuit (in admission order,
+// whole layers, up to the round's bootstrap and gate budget), rewrites their
+// local wire ids to pool slots, and runs them as one batch: key switch ->
+// blind rotation -> combine (engine.hpp). Gates never depend on their
+// batch-mates, so every result is bit-identical to running the circuit
+// alone.
+//
+// Pipelining. Rounds are issued on one stream, so layer l + 1 of a circuit
+// may be issued while layer l still runs: stream order makes it read the
+// finished outputs. Up to two rounds are in flight; each has its own pinned
+// and device job arrays and a pinned output staging area, and two events.
+// While round k runs, the worker admits circuits, builds round k + 1 and
+// issues it; it blocks only when two rounds are in flight or nothing is left
+// to issue. A round retires when its end event completes: the outputs of the
+// circuits whose last layer it ran are copied from the staging area to the
+// caller's buffers, their wire ranges are freed (admitting waiting circuits)
+// and the GPU time between its events is added to the busy time.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -89,14 +99,34 @@ struct Request {
     std::vector<int> layer_end;
     std::vector<int> layer_boot;  // bootstraps per layer
     std::vector<uint32_t> outputs;
-    std::vector<uint32_t> inputs;  // n_inputs x 1025 words
+    uint32_t* inputs = nullptr;  // pinned copy: n_inputs x 1025 words
     uint32_t* host_out = nullptr;
     size_t wires = 0, base = 0;
-    int next_layer = 0, rounds = 0;
+    int next_layer = 0, rounds = 0;  // next_layer: the next layer not yet issued
     int status = 0;  // 0 running, 1 done, < 0 error code
     double t_submit = 0, t_admit = -1, t_done = -1;
+    ~Request() { (void)hipHostFree(inputs); }
 };
 using ReqPtr = std::shared_ptr<Request>;
+
+// Buffers of one in-flight round.
+struct RoundSlot {
+    KsJob* d_ks = nullptr;
+    BrJob* d_br = nullptr;
+    LinJob* d_lin = nullptr;
+    KsJob* h_ks = nullptr;
+    BrJob* h_br = nullptr;
+    LinJob* h_lin = nullptr;
+    uint32_t* h_out = nullptr;  // outputs of the circuits finishing in the round
+    size_t h_out_wires = 0;
+    hipEvent_t ev_start = nullptr, ev_end = nullptr;
+};
+
+struct InFlight {
+    int slot = 0;
+    std::vector<ReqPtr> finishing;
+    std::vector<size_t> out_offset;  // staging offset (wires) of each finishing circuit
+};
 
 }  // namespace
 
@@ -107,14 +137,12 @@ struct cufhe_server {
     int max_boot = 0, c = 1;
     uint64_t key_generation = 0;  // the lvl1 key load the workspaces were sized for
     hipStream_t st = nullptr;
-    KsJob* d_ks = nullptr;
-    BrJob* d_br = nullptr;
-    LinJob* d_lin = nullptr;
+    // stream-ordered, so shared by the rounds in flight
     uint32_t* d_ws = nullptr;
-    void* d_ksws = nullptr;  // GEMM key-switch workspace (max_boot jobs)
-    KsJob* h_ks = nullptr;
-    BrJob* h_br = nullptr;
-    LinJob* h_lin = nullptr;
+    void* d_ksws = nullptr;  // MFMA key-switch workspace (max_boot jobs)
+    RoundSlot slots[2];
+    std::deque<InFlight> inflight;  // worker thread only
+    int next_slot = 0;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<ReqPtr> waiting;
@@ -144,68 +172,159 @@ void fail_all_locked(cufhe_server* S, int code, const std::string& msg)
     S->cv_done.notify_all();
 }
 
-// One round: admit, collect the next layer of every admitted circuit,
-// launch them as one batch, retire finished circuits.
+// Retire the oldest in-flight round once its end event has completed.
+// `block`: wait for it. Returns false if it is still running (block false).
+bool retire_oldest(cufhe_server* S, bool block)
+{
+    InFlight& f = S->inflight.front();
+    RoundSlot& sl = S->slots[f.slot];
+    hipError_t e = block ? hipEventSynchronize(sl.ev_end) : hipEventQuery(sl.ev_end);
+    if (e == hipErrorNotReady) return false;
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, sl.ev_start, sl.ev_end);
+    if (e == hipSuccess) {
+        for (size_t i = 0; i < f.finishing.size(); i++) {
+            const ReqPtr& r = f.finishing[i];
+            std::memcpy(r->host_out, sl.h_out + f.out_offset[i] * kWireWords1,
+                        r->outputs.size() * kWireWords1 * sizeof(uint32_t));
+        }
+    }
+    const double te = S->now();
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (e != hipSuccess) {
+        S->inflight.clear();
+        fail_all_locked(S, CUFHE_ERR_HIP, std::string("server round: ") + hipGetErrorString(e));
+        return true;
+    }
+    S->stats[3] += ms * 1e-3;
+    for (auto& r : f.finishing) {
+        r->status = 1;
+        r->t_done = te;
+        S->arena.release(r->base, r->wires);
+        S->active.erase(std::find(S->active.begin(), S->active.end(), r));
+        // keep the timestamps, drop the circuit
+        std::vector<cufhe_gate_desc>().swap(r->gates);
+        (void)hipHostFree(r->inputs);
+        r->inputs = nullptr;
+    }
+    if (!f.finishing.empty()) S->cv_done.notify_all();
+    S->inflight.pop_front();
+    return true;
+}
+
+// The worker: retire finished rounds, admit, gather the next unscheduled
+// layer of every admitted circuit, issue them as one round; at most two
+// rounds in flight.
 void worker_loop(cufhe_server* S)
 {
     constexpr uint32_t W1 = kWireWords1;
     for (;;) {
+        while (!S->inflight.empty() && retire_oldest(S, false)) {
+        }
         std::vector<ReqPtr> admitted_now, finishing;
         std::vector<cufhe_gate_desc> gates;
+        bool block_on_round = false;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            S->cv_work.wait(lk, [&] { return S->stop || !S->waiting.empty() || !S->active.empty(); });
-            if (S->waiting.empty() && S->active.empty()) {
-                if (S->stop) return;
-                continue;
-            }
-            while (!S->waiting.empty()) {
-                ReqPtr r = S->waiting.front();
-                size_t base = 0;
-                if (!S->arena.alloc(r->wires, &base)) break;
-                r->base = base;
-                r->t_admit = S->now();
-                S->active.push_back(r);
-                admitted_now.push_back(r);
-                S->waiting.pop_front();
-            }
-            S->stats[4] = std::max(S->stats[4], static_cast<double>(S->arena.used()) * W1 * 4);
-            int boots = 0, ng = 0;
-            for (auto& r : S->active) {
-                const int l = r->next_layer;
-                const int g0 = l == 0 ? 0 : r->layer_end[l - 1], g1 = r->layer_end[l];
-                const int lb = r->layer_boot[l], lg = g1 - g0;
-                if (!gates.empty() && (boots + lb > S->max_boot || ng + lg > S->max_boot)) continue;
-                const uint32_t b = static_cast<uint32_t>(r->base);
-                for (int g = g0; g < g1; g++) {
-                    cufhe_gate_desc d = r->gates[g];
-                    d.out = b + static_cast<uint32_t>(r->n_inputs + g);
-                    d.in0 += b;
-                    d.in1 += b;
-                    d.inc += b;
-                    gates.push_back(d);
+            auto issuable = [&] {
+                if (!S->waiting.empty()) return true;
+                for (auto& r : S->active)
+                    if (r->next_layer < r->n_layers) return true;
+                return false;
+            };
+            if (!issuable()) {
+                if (!S->inflight.empty()) {
+                    block_on_round = true;
                 }
-                boots += lb;
-                ng += lg;
-                r->next_layer++;
-                r->rounds++;
-                if (r->next_layer == r->n_layers) finishing.push_back(r);
+                else {
+                    if (S->stop && S->waiting.empty() && S->active.empty()) return;
+                    S->cv_work.wait(lk, [&] { return S->stop || !S->waiting.empty(); });
+                    continue;
+                }
             }
-            S->stats[0] += 1;
-            S->stats[1] += boots;
-            S->stats[2] += ng;
+            if (!block_on_round) {
+                while (!S->waiting.empty()) {
+                    ReqPtr r = S->waiting.front();
+                    size_t base = 0;
+                    if (!S->arena.alloc(r->wires, &base)) break;
+                    r->base = base;
+                    r->t_admit = S->now();
+                    S->active.push_back(r);
+                    admitted_now.push_back(r);
+                    S->waiting.pop_front();
+                }
+                S->stats[4] = std::max(S->stats[4], static_cast<double>(S->arena.used()) * W1 * 4);
+                int boots = 0, ng = 0;
+                for (auto& r : S->active) {
+                    const int l = r->next_layer;
+                    if (l >= r->n_layers) continue;  // every layer issued; retiring
+                    const int g0 = l == 0 ? 0 : r->layer_end[l - 1], g1 = r->layer_end[l];
+                    const int lb = r->layer_boot[l], lg = g1 - g0;
+                    if (!gates.empty() && (boots + lb > S->max_boot || ng + lg > S->max_boot)) continue;
+                    const uint32_t b = static_cast<uint32_t>(r->base);
+                    for (int g = g0; g < g1; g++) {
+                        cufhe_gate_desc d = r->gates[g];
+                        d.out = b + static_cast<uint32_t>(r->n_inputs + g);
+                        d.in0 += b;
+                        d.in1 += b;
+                        d.inc += b;
+                        gates.push_back(d);
+                    }
+                    boots += lb;
+                    ng += lg;
+                    r->next_layer++;
+                    r->rounds++;
+                    if (r->next_layer == r->n_layers) finishing.push_back(r);
+                }
+                if (!gates.empty()) {
+                    S->stats[0] += 1;
+                    S->stats[1] += boots;
+                    S->stats[2] += ng;
+                }
+                else if (!S->inflight.empty()) {
+                    block_on_round = true;  // waiting circuits need the space of running ones
+                }
+            }
         }
-        const double tb = S->now();
+        if (block_on_round) {
+            retire_oldest(S, true);
+            continue;
+        }
+        if (gates.empty()) continue;
         if (lvl1_key_generation() != S->key_generation) {
             std::lock_guard<std::mutex> lk(S->mu);
+            S->inflight.clear();
             fail_all_locked(S, CUFHE_ERR_STATE, "lvl1 keys were reloaded while the server ran");
             continue;
         }
+        // a free slot: at most two rounds in flight
+        if (S->inflight.size() == 2) retire_oldest(S, true);
+        if (S->error) continue;
+        const int si = S->next_slot;
+        S->next_slot ^= 1;
+        RoundSlot& sl = S->slots[si];
+        InFlight f;
+        f.slot = si;
+        size_t out_wires = 0;
+        for (auto& r : finishing) {
+            f.out_offset.push_back(out_wires);
+            out_wires += r->outputs.size();
+        }
+        f.finishing = finishing;
         hipError_t e = hipSuccess;
+        if (out_wires > sl.h_out_wires) {  // the slot is idle: grow its staging
+            (void)hipHostFree(sl.h_out);
+            sl.h_out = nullptr;
+            sl.h_out_wires = 0;
+            e = hipHostMalloc(reinterpret_cast<void**>(&sl.h_out), out_wires * W1 * sizeof(uint32_t),
+                              hipHostMallocDefault);
+            if (e == hipSuccess) sl.h_out_wires = out_wires;
+        }
         for (auto& r : admitted_now) {
             if (e != hipSuccess) break;
-            e = hipMemcpyAsync(S->pool + r->base * W1, r->inputs.data(),
-                               r->inputs.size() * sizeof(uint32_t), hipMemcpyHostToDevice, S->st);
+            e = hipMemcpyAsync(S->pool + r->base * W1, r->inputs,
+                               static_cast<size_t>(r->n_inputs) * W1 * sizeof(uint32_t),
+                               hipMemcpyHostToDevice, S->st);
         }
         Lvl1Jobs jobs;
         jobs.ks.reserve(S->max_boot);
@@ -213,59 +332,59 @@ void worker_loop(cufhe_server* S)
         lvl1_build(gates.data(), static_cast<int>(gates.size()), S->pool, S->d_ws, jobs);
         const int nks = static_cast<int>(jobs.ks.size()), nbr = static_cast<int>(jobs.br.size());
         const int nlin = static_cast<int>(jobs.lin.size());
-        std::memcpy(S->h_ks, jobs.ks.data(), nks * sizeof(KsJob));
-        std::memcpy(S->h_br, jobs.br.data(), nbr * sizeof(BrJob));
-        std::memcpy(S->h_lin, jobs.lin.data(), nlin * sizeof(LinJob));
+        std::memcpy(sl.h_ks, jobs.ks.data(), nks * sizeof(KsJob));
+        std::memcpy(sl.h_br, jobs.br.data(), nbr * sizeof(BrJob));
+        std::memcpy(sl.h_lin, jobs.lin.data(), nlin * sizeof(LinJob));
+        if (e == hipSuccess) e = hipEventRecord(sl.ev_start, S->st);
         if (e == hipSuccess && nks)
-            e = hipMemcpyAsync(S->d_ks, S->h_ks, nks * sizeof(KsJob), hipMemcpyHostToDevice, S->st);
+            e = hipMemcpyAsync(sl.d_ks, sl.h_ks, nks * sizeof(KsJob), hipMemcpyHostToDevice, S->st);
         if (e == hipSuccess && nbr)
-            e = hipMemcpyAsync(S->d_br, S->h_br, nbr * sizeof(BrJob), hipMemcpyHostToDevice, S->st);
+            e = hipMemcpyAsync(sl.d_br, sl.h_br, nbr * sizeof(BrJob), hipMemcpyHostToDevice, S->st);
         if (e == hipSuccess && nlin)
-            e = hipMemcpyAsync(S->d_lin, S->h_lin, nlin * sizeof(LinJob), hipMemcpyHostToDevice, S->st);
+            e = hipMemcpyAsync(sl.d_lin, sl.h_lin, nlin * sizeof(LinJob), hipMemcpyHostToDevice, S->st);
         if (e == hipSuccess) {
             // c = 0: choose per round (small rounds use one job per workgroup)
             const int c = S->c ? S->c : lvl1_pick_c(0, nbr);
-            if (!lvl1_launch(S->d_ks, nks, S->d_ksws, S->d_br, nbr, S->d_lin, nlin, c, S->st))
+            if (!lvl1_launch(sl.d_ks, nks, S->d_ksws, sl.d_br, nbr, sl.d_lin, nlin, c, S->st))
                 e = hipErrorLaunchFailure;
             if (e == hipSuccess) e = hipGetLastError();
         }
-        for (auto& r : finishing) {
+        for (size_t i = 0; i < finishing.size() && e == hipSuccess; i++) {
+            const ReqPtr& r = finishing[i];
             for (size_t o = 0; o < r->outputs.size() && e == hipSuccess; o++)
-                e = hipMemcpyAsync(r->host_out + o * W1, S->pool + (r->base + r->outputs[o]) * W1,
-                                   W1 * sizeof(uint32_t), hipMemcpyDeviceToHost, S->st);
+                e = hipMemcpyAsync(sl.h_out + (f.out_offset[i] + o) * W1,
+                                   S->pool + (r->base + r->outputs[o]) * W1, W1 * sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, S->st);
         }
-        if (e == hipSuccess) e = hipStreamSynchronize(S->st);
-        const double te = S->now();
-        std::lock_guard<std::mutex> lk(S->mu);
-        S->stats[3] += te - tb;
+        if (e == hipSuccess) e = hipEventRecord(sl.ev_end, S->st);
         if (e != hipSuccess) {
+            (void)hipStreamSynchronize(S->st);
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->inflight.clear();
             fail_all_locked(S, CUFHE_ERR_HIP, std::string("server round: ") + hipGetErrorString(e));
             continue;
         }
-        for (auto& r : finishing) {
-            r->status = 1;
-            r->t_done = te;
-            S->arena.release(r->base, r->wires);
-            S->active.erase(std::find(S->active.begin(), S->active.end(), r));
-            // keep the timestamps, drop the circuit
-            std::vector<cufhe_gate_desc>().swap(r->gates);
-            std::vector<uint32_t>().swap(r->inputs);
-        }
-        if (!finishing.empty()) S->cv_done.notify_all();
+        S->inflight.push_back(std::move(f));
     }
 }
 
 void free_server(cufhe_server* S)
 {
+    if (S->st) (void)hipStreamSynchronize(S->st);
     (void)hipFree(S->pool);
-    (void)hipFree(S->d_ks);
-    (void)hipFree(S->d_br);
-    (void)hipFree(S->d_lin);
     (void)hipFree(S->d_ws);
     (void)hipFree(S->d_ksws);
-    (void)hipHostFree(S->h_ks);
-    (void)hipHostFree(S->h_br);
-    (void)hipHostFree(S->h_lin);
+    for (RoundSlot& sl : S->slots) {
+        (void)hipFree(sl.d_ks);
+        (void)hipFree(sl.d_br);
+        (void)hipFree(sl.d_lin);
+        (void)hipHostFree(sl.h_ks);
+        (void)hipHostFree(sl.h_br);
+        (void)hipHostFree(sl.h_lin);
+        (void)hipHostFree(sl.h_out);
+        if (sl.ev_start) (void)hipEventDestroy(sl.ev_start);
+        if (sl.ev_end) (void)hipEventDestroy(sl.ev_end);
+    }
     if (S->st) (void)hipStreamDestroy(S->st);
     delete S;
 }
@@ -292,18 +411,25 @@ int cufhe_server_create(size_t pool_bytes, int max_batch_bootstraps, int jobs_pe
     const size_t mb = static_cast<size_t>(max_batch_bootstraps);
     hipError_t e = hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&S->pool), S->pool_wires * wire_bytes);
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&S->d_ks), mb * sizeof(KsJob));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&S->d_br), mb * sizeof(BrJob));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&S->d_lin), mb * sizeof(LinJob));
+    for (RoundSlot& sl : S->slots) {
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sl.d_ks), mb * sizeof(KsJob));
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sl.d_br), mb * sizeof(BrJob));
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sl.d_lin), mb * sizeof(LinJob));
+        if (e == hipSuccess)
+            e = hipHostMalloc(reinterpret_cast<void**>(&sl.h_ks), mb * sizeof(KsJob), hipHostMallocDefault);
+        if (e == hipSuccess)
+            e = hipHostMalloc(reinterpret_cast<void**>(&sl.h_br), mb * sizeof(BrJob), hipHostMallocDefault);
+        if (e == hipSuccess)
+            e = hipHostMalloc(reinterpret_cast<void**>(&sl.h_lin), mb * sizeof(LinJob), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreate(&sl.ev_start);
+        if (e == hipSuccess) e = hipEventCreate(&sl.ev_end);
+    }
     if (e == hipSuccess)
         e = hipMalloc(reinterpret_cast<void**>(&S->d_ws),
                       lvl1_workspace_words(max_batch_bootstraps, max_batch_bootstraps / 2) *
                           sizeof(uint32_t));
     if (e == hipSuccess && lvl1_ks_workspace_bytes(max_batch_bootstraps))
         e = hipMalloc(&S->d_ksws, lvl1_ks_workspace_bytes(max_batch_bootstraps));
-    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&S->h_ks), mb * sizeof(KsJob), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&S->h_br), mb * sizeof(BrJob), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&S->h_lin), mb * sizeof(LinJob), hipHostMallocDefault);
     if (e != hipSuccess) {
         free_server(S);
         return set_error(CUFHE_ERR_NOMEM, std::string("server allocation: ") + hipGetErrorString(e));
@@ -361,7 +487,14 @@ int cufhe_server_submit(cufhe_server* S, const cufhe_circuit* c, const uint32_t*
     for (uint32_t o : r->outputs)
         if (o >= r->wires) return set_error(CUFHE_ERR_ARG, "output wire out of range");
     if (r->wires > S->pool_wires) return set_error(CUFHE_ERR_SIZE, "circuit larger than the wire pool");
-    r->inputs.assign(host_inputs, host_inputs + static_cast<size_t>(c->n_inputs) * kWireWords1);
+    if (c->n_inputs) {
+        const size_t bytes = static_cast<size_t>(c->n_inputs) * kWireWords1 * sizeof(uint32_t);
+        if (hipHostMalloc(reinterpret_cast<void**>(&r->inputs), bytes, hipHostMallocDefault) != hipSuccess) {
+            r->inputs = nullptr;
+            return set_error(CUFHE_ERR_NOMEM, "pinned input staging allocation failed");
+        }
+        std::memcpy(r->inputs, host_inputs, bytes);
+    }
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->error) return set_error(S->error, S->error_msg);
     if (S->stop) return set_error(CUFHE_ERR_STATE, "server is shutting down");

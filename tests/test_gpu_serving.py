@@ -85,3 +85,53 @@ def test_submit_rejects_bad_circuits(served):
     with pytest.raises(C.CufheError, match="same or a later layer"):
         srv.submit(bad, np.zeros((4, W1), np.uint32))
     srv.destroy()
+
+
+def test_config5_sizes_poisson_alone_equals_mixed(served):
+    """Circuits at the serving workload's sizes (width U[64, 1024], depth
+    U[4, 32], 10% MUX, binary gates uniform) arriving as a Poisson stream:
+    each result is bit-identical to the same circuit run alone, and every
+    output decrypts to the plaintext evaluation. The pipelined worker keeps
+    up to two rounds in flight, so consecutive layers of one circuit are
+    issued back to back."""
+    keys, _, _, _ = served
+    k1 = keys["k1"]
+    rng = np.random.default_rng(505)
+    circuits, inputs, bits = [], [], []
+    for i in range(8):
+        c = random_circuit(rng, int(rng.integers(64, 1025)), int(rng.integers(4, 33)))
+        b = rng.integers(0, 2, c.n_inputs).astype(np.uint8)
+        circuits.append(c)
+        bits.append(b)
+        inputs.append(C.encrypt_bits(1, 700 + i, k1, b))
+    print("circuits (width, depth):", [(c.n_inputs, c.depth) for c in circuits], flush=True)
+    srv = C.Server(pool_bytes=1 << 30)
+    alone = []
+    for c, x in zip(circuits, inputs):
+        alone.append(srv.result(srv.submit(c, x)))
+    srv.destroy()
+    for c, b, out in zip(circuits, bits, alone):
+        assert np.array_equal(C.decrypt_bits(1, k1, out), c.evaluate(b))
+
+    arrivals = np.cumsum(rng.exponential(1.0 / 40.0, len(circuits)))  # 40 circuits/s
+    srv = C.Server(pool_bytes=1 << 30)
+    ids = [None] * len(circuits)
+    t0 = time.perf_counter()
+
+    def client():
+        for i, (c, x) in enumerate(zip(circuits, inputs)):
+            dt = t0 + arrivals[i] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids[i] = srv.submit(c, x)
+
+    t = threading.Thread(target=client)
+    t.start()
+    t.join()
+    mixed = [srv.result(r) for r in ids]
+    st = srv.stats()
+    srv.destroy()
+    for a, m in zip(alone, mixed):
+        assert np.array_equal(a, m)
+    assert st["rounds"] < sum(c.depth for c in circuits)
+    assert 0 < st["busy_s"]
