@@ -274,10 +274,17 @@ def workload_kb4096(args):
     want = np.array([C.TRUTH[o](int(bits[i]), int(bits[B + i])) for i, o in enumerate(ops)],
                     np.uint8)
 
-    cands = [args.jobs_per_group] if args.jobs_per_group else [1, 2, 4, 6, 8]
-    best_c, tune = autotune(C, gates, dpool, cands)
+    if args.jobs_per_group:
+        best_c, tune = autotune(C, gates, dpool, [args.jobs_per_group])
+    else:
+        # the library's autotuner times every rotation kernel on B synthetic
+        # jobs and records the fastest; jobs_per_group 0 then uses it
+        best_c, tune = C.autotune_lvl1(B)
+        for c, t in sorted(tune.items()):
+            log("autotune C=%d: blind rotation %.2f ms" % (c, t))
 
-    plan = C.Plan(gates, dpool, jobs_per_group=best_c)
+    plan = C.Plan(gates, dpool, jobs_per_group=0 if not args.jobs_per_group else best_c)
+    assert plan.jobs_per_group == best_c
     nbr = plan.bootstraps
     log("warmup %d steps (C=%d, %d bootstraps/step)" % (args.warmup, best_c, nbr))
     for _ in range(args.warmup):
@@ -451,8 +458,12 @@ def workload_nttbb(args):
     gates = [(ops[i], 2 * B + i, i, B + i, 0) for i in range(B)]
     want = np.array([C.TRUTH[o](int(bits[i]), int(bits[B + i])) for i, o in enumerate(ops)],
                     np.uint8)
-    cands = [args.jobs_per_group] if args.jobs_per_group else [1, 2, 4, 6]
-    best_c, tune = autotune(C, gates, dpool, cands)
+    if args.jobs_per_group:
+        best_c, tune = autotune(C, gates, dpool, [args.jobs_per_group])
+    else:
+        best_c, tune = C.autotune_lvl1(B)
+        for c, t in sorted(tune.items()):
+            log("autotune C=%d: blind rotation %.2f ms" % (c, t))
     plan = C.Plan(gates, dpool, jobs_per_group=best_c)
     nbr = plan.bootstraps
     for _ in range(args.warmup):

@@ -102,6 +102,7 @@ def lib():
         "cufhe_plan_jobs_per_group": ([vp], i32),
         "cufhe_plan_run_timed": ([vp, vp, ctypes.POINTER(ctypes.c_float)], i32),
         "cufhe_set_key_switch_split": ([i32], i32),
+        "cufhe_autotune_lvl1": ([i32, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_float), vp], i32),
         "cufhe_gate_lvl1": ([i32, vp, vp, vp, vp, vp], i32),
         "cufhe_gate_lvl02": ([i32, vp, vp, vp, vp, vp], i32),
         "cufhe_gate_lvl0": ([i32, vp, vp, vp, vp, vp], i32),
@@ -219,6 +220,16 @@ def encrypt_bits(level, seed, key, bits):
     check(lib().tfhe_tlwe_encrypt_bits(level, seed, _ptr(key), _ptr(bits), len(bits), _ptr(out)),
           "tfhe_tlwe_encrypt_bits")
     return out
+
+
+def autotune_lvl1(nbootstraps, stream=None):
+    """Run the library's rotation autotuner for batches of nbootstraps;
+    returns (best jobs_per_group, {c: rotation ms})."""
+    best = ctypes.c_int32(0)
+    t = (ctypes.c_float * 9)()
+    check(lib().cufhe_autotune_lvl1(int(nbootstraps), ctypes.byref(best), t, _sp(stream)),
+          "cufhe_autotune_lvl1")
+    return best.value, {c: float(t[c]) for c in range(9) if t[c] > 0}
 
 
 def trgsw_encrypt(seed, key1, m, count=1):

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -549,9 +550,36 @@ uint64_t lvl1_key_generation()
     return s.k1.loaded ? s.k1.generation : 0;
 }
 
+// Autotuner results: batch size (bootstraps) -> fastest jobs-per-workgroup
+// choice, for the lvl1 key load `generation` (cufhe_autotune_lvl1).
+struct TuneTable {
+    std::mutex mu;
+    uint64_t generation = 0;
+    std::map<int, int> best;
+};
+TuneTable& tune_table()
+{
+    static TuneTable t;
+    return t;
+}
+
+// Recorded choice for a batch of nbr bootstraps: the entry of the smallest
+// tuned size >= nbr, else the largest tuned size; 0 if none.
+int tuned_c(int nbr)
+{
+    TuneTable& t = tune_table();
+    std::lock_guard<std::mutex> g(t.mu);
+    const LevelKeys<lvl1param>& k = state().k1;
+    if (!k.loaded || t.generation != k.generation || t.best.empty() || nbr <= 0) return 0;
+    auto it = t.best.lower_bound(nbr);
+    if (it == t.best.end()) --it;
+    return it->second;
+}
+
 int lvl1_pick_c(int c, int nbr)
 {
     const LevelKeys<lvl1param>& k = state().k1;
+    if (c == 0) c = tuned_c(nbr);
     if (c == 0) {
         if (k.backend == CUFHE_BACKEND_NTT) {
             // NTT block-binary sweep (4096 gates, BR ms): C=1 18.1, 2 17.3,
@@ -1382,6 +1410,88 @@ int cufhe_gate_batch_lvl1(const cufhe_gate_desc* gates, int n, uint32_t* d_pool,
     }
     cufhe_plan_destroy(p);
     return rc;
+}
+
+int cufhe_autotune_lvl1(int nbootstraps, int* best, float* times_ms, void* stream)
+{
+    State& s = state();
+    if (!s.k1.loaded) return fail(CUFHE_ERR_STATE, "lvl1 keys not loaded");
+    if (nbootstraps <= 0 || nbootstraps > (1 << 20)) return fail(CUFHE_ERR_ARG, "nbootstraps out of range");
+    const hipStream_t st = as_stream(stream);
+    const int n = nbootstraps;
+    constexpr size_t W0 = lvl0param::n + 1, W1 = lvl1param::n + 1;
+    // synthetic jobs: pseudo-random lvl0 inputs (the rotation's work does not
+    // depend on the values), extracted outputs
+    std::vector<uint32_t> h_in(static_cast<size_t>(n) * W0);
+    uint64_t x = 0x9e3779b97f4a7c15ull;
+    for (auto& v : h_in) {
+        x ^= x << 13;
+        x ^= x >> 7;
+        x ^= x << 17;
+        v = static_cast<uint32_t>(x >> 16);
+    }
+    uint32_t* d_in = nullptr;
+    uint32_t* d_out = nullptr;
+    BrJob* d_jobs = nullptr;
+    std::vector<BrJob> jobs(n);
+    struct Mem {
+        void* p[3] = {nullptr, nullptr, nullptr};
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Mem()
+        {
+            for (void* q : p) (void)hipFree(q);
+            for (hipEvent_t v : e)
+                if (v) (void)hipEventDestroy(v);
+        }
+    } mem;
+    CUFHE_HIP_TRY(hipMalloc(&mem.p[0], h_in.size() * 4));
+    CUFHE_HIP_TRY(hipMalloc(&mem.p[1], static_cast<size_t>(n) * W1 * 4));
+    CUFHE_HIP_TRY(hipMalloc(&mem.p[2], static_cast<size_t>(n) * sizeof(BrJob)));
+    d_in = static_cast<uint32_t*>(mem.p[0]);
+    d_out = static_cast<uint32_t*>(mem.p[1]);
+    d_jobs = static_cast<BrJob*>(mem.p[2]);
+    for (int i = 0; i < n; i++)
+        jobs[i] = BrJob{d_in + static_cast<size_t>(i) * W0, nullptr, d_out + static_cast<size_t>(i) * W1,
+                        1, 0, 0, 0};
+    CUFHE_HIP_TRY(hipMemcpyAsync(d_in, h_in.data(), h_in.size() * 4, hipMemcpyHostToDevice, st));
+    CUFHE_HIP_TRY(hipMemcpyAsync(d_jobs, jobs.data(), jobs.size() * sizeof(BrJob), hipMemcpyHostToDevice, st));
+    CUFHE_HIP_TRY(hipEventCreate(&mem.e[0]));
+    CUFHE_HIP_TRY(hipEventCreate(&mem.e[1]));
+    if (times_ms)
+        for (int c = 0; c < 9; c++) times_ms[c] = 0.f;
+    int bestc = 0;
+    float bestt = 0.f;
+    for (int c : {1, 2, 4, 6, 8}) {
+        if (lvl1_pick_c(c, n) != c) continue;  // not valid for these keys
+        if (c == 8 && !s.k1.kb) continue;
+        float t = 0.f;
+        for (int rep = 0; rep < 2; rep++) {  // the first launch warms up
+            CUFHE_HIP_TRY(hipEventRecord(mem.e[0], st));
+            br_lvl1(d_jobs, n, c, static_cast<uint32_t>(lvl1param::μ), st);
+            CUFHE_HIP_TRY(hipEventRecord(mem.e[1], st));
+            const int rc = check_launch("autotune rotation");
+            if (rc != CUFHE_OK) return rc;
+            CUFHE_HIP_TRY(hipEventSynchronize(mem.e[1]));
+            CUFHE_HIP_TRY(hipEventElapsedTime(&t, mem.e[0], mem.e[1]));
+        }
+        if (times_ms) times_ms[c] = t;
+        if (bestc == 0 || t < bestt) {
+            bestc = c;
+            bestt = t;
+        }
+    }
+    if (bestc == 0) return fail(CUFHE_ERR_STATE, "no rotation kernel for these keys");
+    {
+        TuneTable& t = tune_table();
+        std::lock_guard<std::mutex> g(t.mu);
+        if (t.generation != s.k1.generation) {
+            t.best.clear();
+            t.generation = s.k1.generation;
+        }
+        t.best[n] = bestc;
+    }
+    if (best) *best = bestc;
+    return CUFHE_OK;
 }
 
 int cufhe_set_key_switch_split(int split)

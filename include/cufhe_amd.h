@@ -181,6 +181,18 @@ int cufhe_plan_jobs_per_group(const cufhe_plan* plan);
  * wait for it, and return the kernel durations in ms:
  * kernel_ms[0] = key switching, [1] = blind rotation, [2] = combine. */
 int cufhe_plan_run_timed(cufhe_plan* plan, void* stream, float* kernel_ms);
+/* Autotuner of the batched rotation, no reference counterpart: times every
+ * rotation kernel / jobs-per-workgroup choice c valid for the loaded lvl1
+ * keys (1, 2, 4: two waves per job; 6: key chunks shared by six jobs
+ * through LDS; 8 (key bundle): one wave per job, key shared by eight) on
+ * nbootstraps synthetic jobs and records the fastest for that batch size.
+ * Plans, one-shot batches and servers created with jobs_per_group 0 then use
+ * the recorded choice of the smallest tuned size >= their batch (else the
+ * largest tuned size), and the built-in choice when nothing is tuned; a key
+ * load clears the record. best (optional) receives the chosen c;
+ * times_ms (optional, 9 floats) the rotation ms indexed by c (0 = not
+ * valid). Synchronous; allocates scratch for the run. */
+int cufhe_autotune_lvl1(int nbootstraps, int* best, float* times_ms, void* stream);
 /* Tuning knob of the batched (int8 MFMA) key switch, no reference
  * counterpart: the number of waves the digit dimension of each output tile
  * is split over, their partial sums added atomically (1-64); 0 restores the
