@@ -84,6 +84,9 @@ def lib():
         "cufhe_initialize_lvl01_ex": ([vp, sz, i32, i32, vp, sz, i32], i32),
         "cufhe_ntt_transform": ([vp, vp, i32, i32, vp], i32),
         "cufhe_ntt_root": ([], u32),
+        "cufhe_initialize_lvl02_ex": ([vp, sz, i32, i32, vp, sz], i32),
+        "cufhe_ntt64_transform": ([vp, vp, i32, i32, vp], i32),
+        "cufhe_ntt64_root": ([], u64),
         "cufhe_cleanup": ([], i32),
         "cufhe_cleanup_lvl02": ([], i32),
         "cufhe_synchronize": ([], i32),
@@ -207,6 +210,14 @@ def initialize_lvl01(bk, ksk, backend=BACKEND_FFT, variant=BR_KEY_BUNDLE, subset
     ksk = np.ascontiguousarray(ksk, np.uint32)
     check(lib().cufhe_initialize_lvl01_ex(_ptr(bk), bk.size, backend, variant, _ptr(ksk), ksk.size,
                                           int(subset)), "cufhe_initialize_lvl01_ex")
+
+
+def initialize_lvl02(bk, ksk, backend=BACKEND_FFT, key_bundle=True):
+    """Load lvl02 gate keys (polynomial-domain lvl02 BK and lvl20 KSK) on the GPU."""
+    bk = np.ascontiguousarray(bk, np.uint64)
+    ksk = np.ascontiguousarray(ksk, np.uint32)
+    check(lib().cufhe_initialize_lvl02_ex(_ptr(bk), bk.size, backend, int(key_bundle), _ptr(ksk),
+                                          ksk.size), "cufhe_initialize_lvl02_ex")
 
 
 def tlwe_words(level):

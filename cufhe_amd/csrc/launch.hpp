@@ -14,6 +14,8 @@ template <class P>
 struct FourierKeys;
 struct NttKeys;
 struct NttTables;
+struct Ntt64Keys;
+struct Ntt64Tables;
 
 template <class P>
 void launch_to_fourier(cplx* out, const void* in, int npolys, bool integer,
@@ -69,5 +71,17 @@ void launch_gate_lvl1_ntt(int mode, bool subset, bool mux, uint32_t* out, const 
                           const uint32_t* in1, const uint32_t* inc, int32_t ca, int32_t cb,
                           uint32_t offset, int32_t mux_sign, const uint32_t* ksk, const NttKeys& K,
                           hipStream_t st);
+
+// NTT backend (lvl2, Goldilocks prime): plain CMUX or key bundle.
+void launch_to_ntt64(uint64_t* out, const uint64_t* in, int npolys, const Ntt64Tables& tb,
+                     hipStream_t st);
+void launch_ntt64_natural(bool inverse, uint64_t* out, const uint64_t* in, int npolys,
+                          const Ntt64Tables& tb, hipStream_t st);
+void launch_br_ntt64(bool kb, const BrJob* jobs, int njobs, uint64_t mu, const Ntt64Keys& K,
+                     hipStream_t st);
+void launch_gate_lvl02_ntt(bool kb, bool mux, uint32_t* out, const uint32_t* in0,
+                           const uint32_t* in1, const uint32_t* inc, int32_t ca, int32_t cb,
+                           uint32_t offset, int32_t mux_sign, const uint32_t* ksk,
+                           const Ntt64Keys& K, hipStream_t st);
 
 }  // namespace cufhe_amd

@@ -15,6 +15,7 @@
 #include "../../include/tfhe_params.hpp"
 #include "br_fft.hpp"
 #include "br_ntt.hpp"
+#include "br_ntt64.hpp"
 #include "engine.hpp"
 #include "ks_gemm.hpp"
 #include "launch.hpp"
@@ -54,6 +55,12 @@ struct NttTableSet {
     NttTables tb{};
 };
 
+// lvl2 NTT-backend tables (Goldilocks prime), see br_ntt64.hpp.
+struct Ntt64TableSet {
+    uint64_t* mem = nullptr;
+    Ntt64Tables tb{};
+};
+
 template <class P>
 struct LevelKeys {
     bool loaded = false;
@@ -64,10 +71,12 @@ struct LevelKeys {
     bool subset = false;
     cplx* bk = nullptr;
     uint32_t* bk_ntt = nullptr;
+    uint64_t* bk_ntt64 = nullptr;  // lvl2 NTT backend
     uint32_t* ksk = nullptr;
     KsGemmKey ksg;  // byte planes of ksk for the GEMM key switch
     FourierKeys<P> fk{};
     NttKeys nk{};
+    Ntt64Keys n64{};
 };
 
 struct State {
@@ -78,6 +87,7 @@ struct State {
     TableSet<lvl1param> t1;
     TableSet<lvl2param> t2;
     NttTableSet tn;
+    Ntt64TableSet t64;
     LevelKeys<lvl1param> k1;
     LevelKeys<lvl2param> k2;
 };
@@ -198,6 +208,49 @@ void free_ntt_tables(NttTableSet& ts)
     ts = NttTableSet{};
 }
 
+// Goldilocks arithmetic on the host (tables and key constants only).
+uint64_t gold_mulmod(uint64_t a, uint64_t b)
+{
+    return static_cast<uint64_t>(static_cast<unsigned __int128>(a) * b % gold::P);
+}
+uint64_t gold_powmod(uint64_t b, uint64_t e)
+{
+    uint64_t r = 1;
+    for (; e; e >>= 1, b = gold_mulmod(b, b))
+        if (e & 1) r = gold_mulmod(r, b);
+    return r;
+}
+// psi: primitive 4096-th root, ROOT_2_32^(2^32 / 4096)
+uint64_t gold_psi() { return gold_powmod(gold::ROOT_2_32, 1ULL << 20); }
+
+int build_ntt64_tables(Ntt64TableSet& ts)
+{
+    constexpr int N = kNtt64N;
+    // layout: psi_rev[N] psi_inv_rev[N] pw[2N]
+    std::vector<uint64_t> h(4 * N);
+    const uint64_t psi = gold_psi(), psi_inv = gold_powmod(psi, gold::P - 2);
+    for (int i = 0; i < N; i++) {
+        uint32_t b = 0;
+        for (int k = 0; k < 11; k++) b |= ((i >> k) & 1u) << (10 - k);
+        h[i] = gold_powmod(psi, b);
+        h[N + i] = gold_powmod(psi_inv, b);
+    }
+    for (int e = 0; e < 2 * N; e++) h[2 * N + e] = gold_powmod(psi, e);
+    CUFHE_HIP_TRY(hipMalloc(reinterpret_cast<void**>(&ts.mem), h.size() * sizeof(uint64_t)));
+    CUFHE_HIP_TRY(hipMemcpy(ts.mem, h.data(), h.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    ts.tb.psi_rev = ts.mem;
+    ts.tb.psi_inv_rev = ts.mem + N;
+    ts.tb.pw = ts.mem + 2 * N;
+    ts.tb.n_inv = gold_powmod(N, gold::P - 2);
+    return CUFHE_OK;
+}
+
+void free_ntt64_tables(Ntt64TableSet& ts)
+{
+    (void)hipFree(ts.mem);
+    ts = Ntt64TableSet{};
+}
+
 template <class P>
 FourierKeys<P> keys_view(const TableSet<P>& ts, const cplx* bk)
 {
@@ -225,6 +278,7 @@ void free_keys(LevelKeys<P>& k)
 {
     (void)hipFree(k.bk);
     (void)hipFree(k.bk_ntt);
+    (void)hipFree(k.bk_ntt64);
     (void)hipFree(k.ksk);
     ks_gemm_release(k.ksg);
     k = LevelKeys<P>{};
@@ -294,6 +348,8 @@ int ensure_init()
     rc = build_tables(s.t2);
     if (rc != CUFHE_OK) return rc;
     rc = build_ntt_tables(s.tn);
+    if (rc != CUFHE_OK) return rc;
+    rc = build_ntt64_tables(s.t64);
     if (rc != CUFHE_OK) return rc;
     s.initialized = true;
     return CUFHE_OK;
@@ -382,6 +438,44 @@ int load_bk_ntt(LevelKeys<lvl1param>& lk, const NttTableSet& ts, const uint32_t*
     return CUFHE_OK;
 }
 
+// lvl2 key on the NTT backend: torus_to_ntt_mod (Goldilocks) + forward
+// transform per polynomial (reference src/bootstrap_gpu.cu:213-340).
+int load_bk_ntt64(LevelKeys<lvl2param>& lk, const Ntt64TableSet& ts, const uint64_t* bk,
+                  size_t bk_words, bool kb)
+{
+    using P = lvl2param;
+    const size_t ntrgsw = TFHEpp::kBKTRGSWs<TFHEpp::lvl02param>(kb);
+    const size_t want = ntrgsw * TFHEpp::kTRGSWWords<P>;
+    if (bk_words != want)
+        return fail(CUFHE_ERR_SIZE, "bootstrapping key has " + std::to_string(bk_words) +
+                                        " words, expected " + std::to_string(want));
+    const size_t npolys = ntrgsw * TFHEpp::BootstrappingTRGSWRows<P> * (P::k + 1);
+    void* tmp = nullptr;
+    CUFHE_HIP_TRY(hipMalloc(&tmp, want * sizeof(uint64_t)));
+    hipError_t e = hipMemcpy(tmp, bk, want * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(tmp);
+        return fail(CUFHE_ERR_HIP, std::string("bk upload: ") + hipGetErrorString(e));
+    }
+    e = hipMalloc(reinterpret_cast<void**>(&lk.bk_ntt64), npolys * P::n * sizeof(uint64_t));
+    if (e != hipSuccess) {
+        (void)hipFree(tmp);
+        return fail(CUFHE_ERR_NOMEM, "bootstrapping key NTT allocation failed");
+    }
+    launch_to_ntt64(lk.bk_ntt64, static_cast<const uint64_t*>(tmp), static_cast<int>(npolys), ts.tb,
+                    nullptr);
+    e = hipDeviceSynchronize();
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(CUFHE_ERR_HIP, std::string("bk conversion: ") + hipGetErrorString(e));
+    lk.n64.bk = lk.bk_ntt64;
+    lk.n64.tb = ts.tb;
+    // gadget identity h_q = 2^(64 - (q+1) Bgbit) as torus_to_ntt_mod
+    for (int q = 0; q < static_cast<int>(P::l); q++)
+        lk.n64.one[q] = torus64_to_gold(1ULL << (64 - (q + 1) * P::Bgbit));
+    lk.kb = kb;
+    return CUFHE_OK;
+}
+
 int load_ksk(uint32_t** dst, const uint32_t* ksk, size_t words, size_t want)
 {
     if (words != want)
@@ -449,6 +543,32 @@ bool ks_lvl1(const KsJob* jobs, int n, void* ws, hipStream_t st)
     if (ws && n >= kKsGemmMinJobs && k.ksg.planes) return ks_gemm_run(k.ksg, jobs, n, ws, st) == 0;
     launch_ks_batch(1, k.subset, jobs, n, k.ksk, st);
     return true;
+}
+
+// Batched lvl2 blind rotation (+ SampleExtract or the TRLWE) on the loaded
+// lvl2 backend.
+void br_lvl2(const BrJob* jobs, int n, uint64_t mu, hipStream_t st)
+{
+    const LevelKeys<lvl2param>& k = state().k2;
+    if (n <= 0) return;
+    if (k.backend == CUFHE_BACKEND_NTT)
+        launch_br_ntt64(k.kb, jobs, n, mu, k.n64, st);
+    else if (k.kb)
+        launch_br_fft<lvl2param, true>(jobs, n, 1, mu, k.fk, st);
+    else
+        launch_br_fft<lvl2param, false>(jobs, n, 1, mu, k.fk, st);
+}
+
+// One lvl02 gate (BR-IKS) as a single launch on the loaded lvl2 backend.
+void gate_lvl02_one(bool mux, uint32_t* out, const uint32_t* in0, const uint32_t* in1,
+                    const uint32_t* inc, int32_t ca, int32_t cb, uint32_t offset, int32_t mux_sign,
+                    hipStream_t st)
+{
+    const LevelKeys<lvl2param>& k = state().k2;
+    if (k.backend == CUFHE_BACKEND_NTT)
+        launch_gate_lvl02_ntt(k.kb, mux, out, in0, in1, inc, ca, cb, offset, mux_sign, k.ksk, k.n64, st);
+    else
+        launch_gate_lvl02_fft(k.kb, mux, out, in0, in1, inc, ca, cb, offset, mux_sign, k.ksk, k.fk, st);
 }
 
 bool ks_lvl2(const KsJob* jobs, int n, void* ws, hipStream_t st)
@@ -669,14 +789,7 @@ int plan_launch(cufhe_plan* p, hipStream_t st)
     if (p->level == 2) {
         // BR-IKS order: rotations (pre-added lvl0 inputs) -> lvl2 TLWEs ->
         // key switch (MUX halves combined in its pre-add)
-        if (p->njobs_br) {
-            if (s.k2.kb)
-                launch_br_fft<lvl2param, true>(p->br, p->njobs_br, 1,
-                                               static_cast<uint64_t>(lvl2param::μ), s.k2.fk, st);
-            else
-                launch_br_fft<lvl2param, false>(p->br, p->njobs_br, 1,
-                                                static_cast<uint64_t>(lvl2param::μ), s.k2.fk, st);
-        }
+        br_lvl2(p->br, p->njobs_br, static_cast<uint64_t>(lvl2param::μ), st);
         if (!ks_lvl2(p->ks, p->njobs_ks, p->ks_ws, st))
             return fail(CUFHE_ERR_HIP, "GEMM key switch failed");
         if (p->njobs_lin) launch_linear(p->lin, p->njobs_lin, st);
@@ -782,13 +895,24 @@ int cufhe_initialize_lvl01_ex(const uint32_t* bk, size_t bk_words, int backend, 
 int cufhe_initialize_lvl02(const uint64_t* bk, size_t bk_words, int key_bundle,
                            const uint32_t* ksk, size_t ksk_words)
 {
+    return cufhe_initialize_lvl02_ex(bk, bk_words, CUFHE_BACKEND_FFT, key_bundle, ksk, ksk_words);
+}
+
+int cufhe_initialize_lvl02_ex(const uint64_t* bk, size_t bk_words, int backend, int key_bundle,
+                              const uint32_t* ksk, size_t ksk_words)
+{
     if (!bk || !ksk) return fail(CUFHE_ERR_ARG, "null key pointer");
+    if (backend != CUFHE_BACKEND_FFT && backend != CUFHE_BACKEND_NTT)
+        return fail(CUFHE_ERR_ARG, "unknown backend " + std::to_string(backend));
     State& s = state();
     std::lock_guard<std::mutex> g(s.mu);
     int rc = ensure_init();
     if (rc != CUFHE_OK) return rc;
     free_keys(s.k2);
-    rc = load_bk<TFHEpp::lvl02param>(s.k2, s.t2, bk, bk_words, key_bundle != 0);
+    if (backend == CUFHE_BACKEND_NTT)
+        rc = load_bk_ntt64(s.k2, s.t64, bk, bk_words, key_bundle != 0);
+    else
+        rc = load_bk<TFHEpp::lvl02param>(s.k2, s.t2, bk, bk_words, key_bundle != 0);
     if (rc == CUFHE_OK) rc = load_ksk(&s.k2.ksk, ksk, ksk_words, TFHEpp::kKSKWords<TFHEpp::lvl20param>);
     if (rc == CUFHE_OK && ks_gemm_prepare(s.k2.ksg, s.k2.ksk, 2, false, nullptr) != 0)
         rc = fail(CUFHE_ERR_HIP, "key-switching key byte planes");
@@ -796,6 +920,7 @@ int cufhe_initialize_lvl02(const uint64_t* bk, size_t bk_words, int key_bundle,
         free_keys(s.k2);
         return rc;
     }
+    s.k2.backend = backend;
     s.k2.generation = ++s.generations;
     s.k2.loaded = true;
     return CUFHE_OK;
@@ -812,6 +937,7 @@ int cufhe_cleanup(void)
     free_tables(s.t1);
     free_tables(s.t2);
     free_ntt_tables(s.tn);
+    free_ntt64_tables(s.t64);
     s.initialized = false;
     return CUFHE_OK;
 }
@@ -955,15 +1081,14 @@ int cufhe_gate_lvl02(int op, uint32_t* out, const uint32_t* in0, const uint32_t*
     }
     if (op == CUFHE_MUX || op == CUFHE_NMUX) {
         if (!in1 || !inc) return fail(CUFHE_ERR_ARG, "null MUX operand");
-        launch_gate_lvl02_fft(s.k2.kb, true, out, in0, in1, inc, 0, 0, 0u,
-                              op == CUFHE_MUX ? 1 : -1, s.k2.ksk, s.k2.fk, st);
+        gate_lvl02_one(true, out, in0, in1, inc, 0, 0, 0u, op == CUFHE_MUX ? 1 : -1, st);
         return check_launch("mux gate lvl02");
     }
     GateLin gl;
     if (!gate_lin(op, gl)) return fail(CUFHE_ERR_ARG, "unknown gate op " + std::to_string(op));
     if (!in1) return fail(CUFHE_ERR_ARG, "null second operand");
-    launch_gate_lvl02_fft(s.k2.kb, false, out, in0, in1, nullptr, gl.ca, gl.cb,
-                          static_cast<uint32_t>(gl.off_mu) * mu0, 0, s.k2.ksk, s.k2.fk, st);
+    gate_lvl02_one(false, out, in0, in1, nullptr, gl.ca, gl.cb, static_cast<uint32_t>(gl.off_mu) * mu0,
+                   0, st);
     return check_launch("gate lvl02");
 }
 
@@ -1340,14 +1465,7 @@ int cufhe_plan_run_timed(cufhe_plan* p, void* stream, float* kernel_ms)
     hipEvent_t* e = evs.e;
     if (p->level == 2) {
         CUFHE_HIP_TRY(hipEventRecord(e[0], st));
-        if (p->njobs_br) {
-            if (s.k2.kb)
-                launch_br_fft<lvl2param, true>(p->br, p->njobs_br, 1,
-                                               static_cast<uint64_t>(lvl2param::μ), s.k2.fk, st);
-            else
-                launch_br_fft<lvl2param, false>(p->br, p->njobs_br, 1,
-                                                static_cast<uint64_t>(lvl2param::μ), s.k2.fk, st);
-        }
+        br_lvl2(p->br, p->njobs_br, static_cast<uint64_t>(lvl2param::μ), st);
         CUFHE_HIP_TRY(hipEventRecord(e[1], st));
         if (!ks_lvl2(p->ks, p->njobs_ks, p->ks_ws, st))
             return fail(CUFHE_ERR_HIP, "key switch failed to launch");
@@ -1603,10 +1721,7 @@ int cufhe_bootstrap_lvl02(uint64_t* d_out, const uint32_t* d_in, uint64_t mu, in
     BrJob* d = nullptr;
     CUFHE_HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&d), n * sizeof(BrJob), st));
     CUFHE_HIP_TRY(hipMemcpyAsync(d, jobs.data(), n * sizeof(BrJob), hipMemcpyHostToDevice, st));
-    if (s.k2.kb)
-        launch_br_fft<lvl2param, true>(d, n, 1, mu, s.k2.fk, st);
-    else
-        launch_br_fft<lvl2param, false>(d, n, 1, mu, s.k2.fk, st);
+    br_lvl2(d, n, mu, st);
     CUFHE_HIP_TRY(hipFreeAsync(d, st));
     CUFHE_HIP_TRY(hipStreamSynchronize(st));
     return check_launch("bootstrap lvl02");
@@ -1628,6 +1743,23 @@ int cufhe_ntt_transform(const uint32_t* d_in, uint32_t* d_out, int count, int in
 }
 
 uint32_t cufhe_ntt_root(void) { return ntt31::PSI; }
+
+int cufhe_ntt64_transform(const uint64_t* d_in, uint64_t* d_out, int count, int inverse,
+                          void* stream)
+{
+    State& s = state();
+    {
+        std::lock_guard<std::mutex> g(s.mu);
+        const int rc = ensure_init();
+        if (rc != CUFHE_OK) return rc;
+    }
+    if (count < 0 || !d_in || !d_out) return fail(CUFHE_ERR_ARG, "bad arguments");
+    if (count == 0) return CUFHE_OK;
+    launch_ntt64_natural(inverse != 0, d_out, d_in, count, s.t64.tb, as_stream(stream));
+    return check_launch("ntt64 transform");
+}
+
+uint64_t cufhe_ntt64_root(void) { return gold_psi(); }
 
 int cufhe_ffnt_forward(int N, const void* d_poly, int integer, void* d_spectra, int count,
                        void* stream)

@@ -105,6 +105,13 @@ int cufhe_initialize_lvl01_ex(const uint32_t* bk, size_t bk_words, int backend, 
  * KeySwitchingKey<lvl20param>. */
 int cufhe_initialize_lvl02(const uint64_t* bk, size_t bk_words, int key_bundle,
                            const uint32_t* ksk, size_t ksk_words);
+/* Initialize_lvl02 with an explicit backend for the lvl2 blind rotation:
+ * CUFHE_BACKEND_FFT (FP64 FFNT) or CUFHE_BACKEND_NTT (exact arithmetic mod the
+ * Goldilocks prime, reference USE_NTT with lvl2 in CMakeLists.txt:18-48,
+ * src/bootstrap_gpu.cu:213-340, :356-366); key_bundle selects the key layout as in
+ * cufhe_initialize_lvl02, which is this call with CUFHE_BACKEND_FFT. */
+int cufhe_initialize_lvl02_ex(const uint64_t* bk, size_t bk_words, int backend, int key_bundle,
+                              const uint32_t* ksk, size_t ksk_words);
 /* CleanUp() / CleanUp_lvl02(). */
 int cufhe_cleanup(void);
 int cufhe_cleanup_lvl02(void);
@@ -319,6 +326,15 @@ int cufhe_ffnt_inverse(int N, const void* d_spectra, double* d_poly, int count, 
 int cufhe_ntt_transform(const uint32_t* d_in, uint32_t* d_out, int count, int inverse,
                         void* stream);
 uint32_t cufhe_ntt_root(void);
+/* Negacyclic NTT mod the Goldilocks prime P = 2^64 - 2^32 + 1 of count
+ * polynomials of N = 2048 residues in [0, P), the lvl2 NTT backend's
+ * transform: out[k] = sum_i in[i] psi^(i (2k+1)) mod P in natural bin order
+ * (psi: primitive 4096-th root, reported by cufhe_ntt64_root()); inverse != 0
+ * maps such spectra back (incl. N^-1) (reference include/ntt_small_modulus.cuh
+ * :60-74, :154-193, src/ntt_small_modulus.cu:85-113). */
+int cufhe_ntt64_transform(const uint64_t* d_in, uint64_t* d_out, int count, int inverse,
+                          void* stream);
+uint64_t cufhe_ntt64_root(void);
 /* ---- CMUX / TRGSW API (lvl1, the loaded lvl1 backend) ---------------------
  * TRGSW2NTT (reference src/bootstrap_gpu.cu:165-186, include/cufhe_gpu.cuh:
  * 219): count lvl1 TRGSWs in the polynomial domain (TFHEpp layout, 4 rows x

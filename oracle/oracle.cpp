@@ -544,11 +544,13 @@ const u64* xai_ntt(u32 N, u32 a)
         std::vector<u64> c(N);
         for (u32 e = 0; e < 2 * N; e++) {
             std::fill(c.begin(), c.end(), 0);
+            // X^e - 1 (X^e = -X^(e-N) for e >= N); field ops, as c[0] = P - 1
+            // plus P - 1 overflows 64 bits for the Goldilocks prime (e = N)
             c[0] = Pm - 1;
             if (e < N)
-                c[e] = (c[e] + 1) % Pm;
+                c[e] = nt.f.add(c[e], 1);
             else
-                c[e - N] = (c[e - N] + Pm - 1) % Pm;
+                c[e - N] = nt.f.sub(c[e - N], 1);
             nt.forward(c);
             std::copy(c.begin(), c.end(), t.v.begin() + static_cast<size_t>(e) * N);
         }

@@ -4,7 +4,7 @@ that each pin catches a planted mistake. CPU only.
 Pinned here: the blind-rotation index arithmetic (body truncation, rounded
 masks, bar = 2N - ms(b), the test vector, the key-bundle pairing, the
 block-binary body correction with correction != 0) in the exact, FFT and NTT
-modes; the identity key switch's rounding (lvl10, subset lvl10, lvl20); and
+modes, at lvl1 and (with masks that round to a = N and a0 + a1 = N) at lvl2; the identity key switch's rounding (lvl10, subset lvl10, lvl20); and
 ntt_mod_to_torus32 over every residue.
 """
 import numpy as np
@@ -58,6 +58,38 @@ def test_noiseless_rotation_is_the_closed_form(variant, mode):
             assert d < NTT_TOL, d
         else:
             assert d == 0, d
+
+
+# lvl2 (N = 2048, Torus64): Z_P rounding of the gadget entries stays far below
+# the 2 mu = 2^62 step of an index error
+NTT2_TOL = 1 << 40
+FFT2_TOL = 1 << 40
+
+
+def _sample_lvl2_edges(rng):
+    """A lvl0 sample whose first mask pair rounds to a0 + a1 = N (X^N - 1 = -2
+    in the key bundle) and whose third mask rounds to a = N."""
+    tl0 = _sample(rng)
+    unit = (1 << 32) // (2 * Q.N2)
+    tl0[0], tl0[1], tl0[2] = 1000 * unit, 1048 * unit, Q.N2 * unit
+    assert Q.ms_round(tl0[0], Q.N2) + Q.ms_round(tl0[1], Q.N2) == Q.N2
+    return tl0
+
+
+@pytest.mark.parametrize("variant,mode", [("plain", "ntt"), ("kb", "ntt"), ("kb", "fft"),
+                                          ("kb", "exact")])
+def test_noiseless_rotation_is_the_closed_form_lvl2(variant, mode):
+    rng = np.random.default_rng(150 + list(VARIANTS).index(variant))
+    s0 = _key(rng, variant)
+    s0[0] = s0[1] = s0[2] = 1  # the edge masks contribute
+    bk = Q.noiseless_bk(s0, variant, n=Q.N2, l=4, bgbit=10, w=64)
+    tl0 = _sample_lvl2_edges(rng)
+    got = O.blind_rotate(2, MODES[mode], VARIANTS[variant], tl0, bk, Q.MU2)
+    want = Q.rotated_test_vector(Q.rotation_amount(tl0, s0, variant, n=Q.N2), n=Q.N2, mu=Q.MU2,
+                                 w=64)
+    d = int(np.max(np.abs((got - want).view(np.int64).astype(np.float64))))
+    tol = {"ntt": NTT2_TOL, "fft": FFT2_TOL, "exact": 0}[mode]
+    assert d <= tol, d
 
 
 def test_block_binary_samples_have_nonzero_correction():
