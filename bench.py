@@ -9,8 +9,9 @@ batch with inputs resident in HBM: identity key switching of all gates, then
 the fused blind-rotation + sample-extraction kernel. Keys from seed 0.
 
 Other workloads (--workload), each printing one JSON line of the same shape:
-  lvl2mix  batched lvl02 gates (N=2048, Torus64, KB=2, FFT): 1536 binary +
-           512 MUX (2 bootstraps each), per-gate-type throughput vs the plain
+  lvl2mix  batched lvl02 gates (N=2048, Torus64, KB=2, FFT, or with
+           --lvl2-backend ntt the Goldilocks NTT): 1536 binary + 512 MUX
+           (2 bootstraps each), per-gate-type throughput vs the plain
            per-gate form;
   nttbb    NTT backend, block-binary lvl0 key, subset key switching: 4096
            NAND/XOR gates, bit-exact against the oracle on a sample.
@@ -42,6 +43,10 @@ VALU_PEAK_GINST = 256 * 2.4
 # VALU wave-instructions of one 4096-gate k_br_ntt<2, 2> launch (rocprofv3
 # SQ_INSTS_VALU, profiles/br_ntt_bb_lvl1_batch4096_counters.md)
 NTT_BB_VALU_INSTS_C2 = 7.609e9
+# VALU wave-instructions per lvl2 key-bundle bootstrap of k_br_ntt64<true>
+# (rocprofv3 SQ_INSTS_VALU over a 2560-rotation launch,
+# profiles/br_ntt64_kb_lvl2_mix2048_counters.md)
+NTT64_KB_VALU_INSTS_PER_BOOTSTRAP = 5.281e7
 GATE_MIX = ["NAND", "AND", "OR", "XOR", "XNOR", "NOR"]
 
 
@@ -357,13 +362,13 @@ def workload_lvl2mix(args):
     import cufhe_amd as C
 
     nbin, nmux = 1536, 512
-    log("keys (seed 0, lvl02 key bundle)")
+    ntt = args.lvl2_backend == "ntt"
+    log("keys (seed 0, lvl02 key bundle, %s backend)" % args.lvl2_backend)
     k0, _, k2 = C.secret_key_gen(0)
     bk = C.bk_gen(2, True, 0, k0, k2)
     ksk = C.ksk_gen(2, 0, k2, k0)
     C.check(C.lib().cufhe_initialize(), "initialize")
-    C.check(C.lib().cufhe_initialize_lvl02(C._ptr(bk), bk.size, 1, C._ptr(ksk), ksk.size),
-            "initialize_lvl02")
+    C.initialize_lvl02(bk, ksk, backend=C.BACKEND_NTT if ntt else C.BACKEND_FFT, key_bundle=True)
     rng = np.random.default_rng(args.seed)
     ops = [GATE_MIX[i] for i in rng.integers(0, len(GATE_MIX), nbin)] + ["MUX"] * nmux
     perm = rng.permutation(len(ops))
@@ -412,10 +417,23 @@ def workload_lvl2mix(args):
     mux_idx = [i for i in range(n) if ops[i] == "MUX"][:sample]
     pb, pb_lat = plain_form(C, gate_fn, [(i,) for i in bin_idx], args.plain_streams)
     pm, pm_lat = plain_form(C, gate_fn, [(i,) for i in mux_idx], args.plain_streams)
-    achieved = FLOP_PER_BOOTSTRAP_KB_LVL2 * nb_mix / (br_mix_ms / 1e3) / 1e12
-    res = base_result(args, mix["gates_per_s"], mix["ms_per_step"], "f64", {
-        "workload": "batched lvl02 gates (lvl0 in/out, BR at N=2048 Torus64, KB=2, FFT): "
-                    "1536 binary (uniform over NAND/AND/OR/XOR/XNOR/NOR) + 512 MUX, interleaved",
+    if ntt:
+        # integer-only kernel bound by VALU issue: measured VALU wave-instructions
+        # per bootstrap over the live launch time
+        valu_g = NTT64_KB_VALU_INSTS_PER_BOOTSTRAP * nb_mix / (br_mix_ms / 1e3) / 1e9
+        roof = {"bound": "valu", "achieved": round(valu_g, 1), "peak": VALU_PEAK_GINST,
+                "unit": "G wave-instr/s", "frac": round(valu_g / VALU_PEAK_GINST, 4),
+                "traffic": measured_traffic("lvl2mix_ntt"),
+                "basis": "measured SQ_INSTS_VALU per bootstrap (no FP64; integer issue bound)"}
+    else:
+        achieved = FLOP_PER_BOOTSTRAP_KB_LVL2 * nb_mix / (br_mix_ms / 1e3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                "traffic": measured_traffic("lvl2mix")}
+    res = base_result(args, mix["gates_per_s"], mix["ms_per_step"], "u64 mod P" if ntt else "f64", {
+        "workload": "batched lvl02 gates (lvl0 in/out, BR at N=2048 Torus64, KB=2, %s): "
+                    "1536 binary (uniform over NAND/AND/OR/XOR/XNOR/NOR) + 512 MUX, interleaved"
+                    % ("NTT mod 2^64-2^32+1" if ntt else "FFT"),
         "global_batch": n, "n": 636, "N": 2048, "l": 4, "Bgbit": 10,
         "parallelism": "single GPU",
     })
@@ -428,9 +446,7 @@ def workload_lvl2mix(args):
                        "mux_latency_p50_ms": round(pm_lat, 3), "streams": args.plain_streams},
         "batched_over_plain": {"binary": round(binr["gates_per_s"] / pb, 2),
                                "mux": round(muxr["gates_per_s"] / pm, 2)},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": measured_traffic("lvl2mix")},
+        "roofline": roof,
         "cpu_baseline": None,
     })
     return res
@@ -640,6 +656,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="kb4096")
     ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--lvl2-backend", choices=["fft", "ntt"], default="fft",
+                    help="lvl2mix: blind rotation on the FP64 FFT or the Goldilocks NTT backend")
     ap.add_argument("--jobs-per-group", type=int, default=0,
                     help="bootstraps per workgroup; 0 = autotune over {1, 2, 4, 6, 8} "
                          "(6: key shared through LDS, FFT key bundle only)")
