@@ -6,7 +6,7 @@
 // CuSafeCall does (include/error_gpu.cuh:30-66).
 //
 // Differences a caller can see: one GPU per process (SetGPUNum(n > 1) is
-// refused); the backend (FFT / NTT block-binary) follows the EvalKey instead
+// refused); the backend (FFT / NTT, block-binary) follows the EvalKey instead
 // of a build option.
 #pragma once
 
@@ -49,7 +49,8 @@ inline void Initialize(const TFHEpp::EvalKey& ek)
     }
     // reference build options as key properties: block-binary keys run on the
     // NTT backend (USE_BLOCK_BINARY implies USE_FFT=OFF), others on the FFT one
-    const int backend = ek.block_binary ? CUFHE_BACKEND_NTT : CUFHE_BACKEND_FFT;
+    // unless ek.ntt asks for the NTT backend (USE_FFT=OFF)
+    const int backend = (ek.block_binary || ek.ntt) ? CUFHE_BACKEND_NTT : CUFHE_BACKEND_FFT;
     const int variant = ek.block_binary ? CUFHE_BR_BLOCK_BINARY
                                         : (ek.key_bundle ? CUFHE_BR_KEY_BUNDLE : CUFHE_BR_PLAIN);
     CuCheck(cufhe_initialize_lvl01_ex(ek.bklvl01->data(), ek.bklvl01->size(), backend, variant,
@@ -64,8 +65,10 @@ inline void Initialize_lvl02(const TFHEpp::EvalKey& ek, const TFHEpp::SecretKey&
         std::fprintf(stderr, "cufhe: Initialize_lvl02 needs bklvl02 and iksklvl20\n");
         std::exit(-1);
     }
-    CuCheck(cufhe_initialize_lvl02(ek.bklvl02->data(), ek.bklvl02->size(), ek.key_bundle,
-                                   ek.iksklvl20->data(), ek.iksklvl20->size()),
+    // ek.ntt: the lvl2 rotation on the Goldilocks NTT backend (USE_FFT=OFF)
+    CuCheck(cufhe_initialize_lvl02_ex(ek.bklvl02->data(), ek.bklvl02->size(),
+                                      ek.ntt ? CUFHE_BACKEND_NTT : CUFHE_BACKEND_FFT,
+                                      ek.key_bundle, ek.iksklvl20->data(), ek.iksklvl20->size()),
             "Initialize_lvl02");
 }
 

@@ -75,6 +75,7 @@ struct EvalKey {
     bool key_bundle = true;    // 2-bit key bundle (reference default build)
     bool block_binary = false;  // block-binary rotation (NTT backend; reference USE_BLOCK_BINARY)
     bool subset_iksk = false;   // iksklvl10 holds the subset key (emplacesubiksk)
+    bool ntt = false;           // NTT transform backend at both levels (reference USE_FFT=OFF)
     EvalKey() = default;
     // Keys for sk's configuration: a block-binary secret key selects the
     // block-binary rotation (one TRGSW per key bit, no key bundle).

@@ -154,6 +154,32 @@ int main()
         bad += report("XxxBootstrap<lvl02param, mu, lvl20param>", e2, total);
     }
 
+    // the same lvl02 launchers with the lvl2 rotation on the NTT backend
+    // (EvalKey::ntt, the reference's USE_FFT=OFF build), then back to FFT
+    {
+        ek.ntt = true;
+        cufhe::Initialize_lvl02(ek, sk);
+        int e = 0, total = 0;
+        for (int gi : {0, 8})  // Nand, Xor
+            for (int x = 0; x < 2; x++)
+                for (int y = 0; y < 2; y++) {
+                    const G& g = gates[gi];
+                    auto c0x = tlweSymEncrypt<lvl0param>(x ? lvl0param::μ : -lvl0param::μ,
+                                                         lvl0param::α, sk.key.lvl0);
+                    auto c0y = tlweSymEncrypt<lvl0param>(y ? lvl0param::μ : -lvl0param::μ,
+                                                         lvl0param::α, sk.key.lvl0);
+                    uint32_t *dx0 = upload<lvl0param>(c0x), *dy0 = upload<lvl0param>(c0y);
+                    uint32_t* o2 = dev_alloc<lvl0param>(lvl0param::n + 1);
+                    g.br02(o2, dx0, dy0, st, 0);
+                    e += tlweSymDecrypt<lvl0param>(download<lvl0param>(o2), sk.key.lvl0) != g.plain(x, y);
+                    total++;
+                    for (uint32_t* p : {dx0, dy0, o2}) cufhe_device_free(p);
+                }
+        bad += report("XxxBootstrap<lvl02param, mu, lvl20param> on the NTT backend", e, total);
+        ek.ntt = false;
+        cufhe::Initialize_lvl02(ek, sk);
+    }
+
     // Mux / NMux / Copy / Not launchers
     {
         int e = 0, total = 0;
