@@ -369,6 +369,8 @@ def workload_lvl2mix(args):
     ksk = C.ksk_gen(2, 0, k2, k0)
     C.check(C.lib().cufhe_initialize(), "initialize")
     C.initialize_lvl02(bk, ksk, backend=C.BACKEND_NTT if ntt else C.BACKEND_FFT, key_bundle=True)
+    if args.lvl2_jobs_per_group:
+        C.check(C.lib().cufhe_set_lvl2_jobs_per_group(args.lvl2_jobs_per_group), "lvl2 jobs")
     rng = np.random.default_rng(args.seed)
     ops = [GATE_MIX[i] for i in rng.integers(0, len(GATE_MIX), nbin)] + ["MUX"] * nmux
     perm = rng.permutation(len(ops))
@@ -658,6 +660,8 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lvl2-backend", choices=["fft", "ntt"], default="fft",
                     help="lvl2mix: blind rotation on the FP64 FFT or the Goldilocks NTT backend")
+    ap.add_argument("--lvl2-jobs-per-group", type=int, default=0,
+                    help="lvl2mix, FFT backend: lvl2 jobs per workgroup (1 or 2; 0 = library default)")
     ap.add_argument("--jobs-per-group", type=int, default=0,
                     help="bootstraps per workgroup; 0 = autotune over {1, 2, 4, 6, 8} "
                          "(6: key shared through LDS, FFT key bundle only)")

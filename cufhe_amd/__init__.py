@@ -87,6 +87,7 @@ def lib():
         "cufhe_initialize_lvl02_ex": ([vp, sz, i32, i32, vp, sz], i32),
         "cufhe_ntt64_transform": ([vp, vp, i32, i32, vp], i32),
         "cufhe_ntt64_root": ([], u64),
+        "cufhe_set_lvl2_jobs_per_group": ([i32], i32),
         "cufhe_cleanup": ([], i32),
         "cufhe_cleanup_lvl02": ([], i32),
         "cufhe_synchronize": ([], i32),

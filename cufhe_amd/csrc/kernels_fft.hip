@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void k_from_fourier(double* out, const cplx* i
 // key reads then meet in L2 instead of drifting apart as later workgroups
 // start when earlier ones finish.
 template <class P, bool KB, int C>
-__global__ __launch_bounds__(128 * C, CUFHE_BR_WAVES_PER_SIMD) void k_br_fft(const BrJob* __restrict__ jobs, int njobs,
+__global__ __launch_bounds__(128 * C, (sizeof(typename P::T) == 8 && C > 1) ? 1 : CUFHE_BR_WAVES_PER_SIMD) void k_br_fft(const BrJob* __restrict__ jobs, int njobs,
                                                      typename P::T mu, FourierKeys<P> K)
 {
     using T = typename P::T;
@@ -779,11 +779,18 @@ void launch_br_fft(const BrJob* jobs, int njobs, int c, typename P::T mu,
             hipLaunchKernelGGL((k_br_fft<P, KB, 1>), g1, block, 0, st, jobs, njobs, mu, K);
             break;
         }
-        case 2:
-            if constexpr (sizeof(typename P::T) == 4)
-                hipLaunchKernelGGL((k_br_fft<P, KB, 2>), grid, block, 0, st, jobs, njobs, mu,
-                                   K);
+        case 2: {
+            // Torus64: two jobs (136 KiB LDS) per workgroup in a resident grid,
+            // their key loads issued together (a tuning option; one job per
+            // workgroup measured faster on the lvl02 mix)
+            dim3 g2 = grid;
+            if constexpr (sizeof(typename P::T) == 8) {
+                const int cap = br_fft_resident_groups<P, KB, 2>();
+                if (cap > 0 && static_cast<int>(g2.x) > cap) g2.x = cap;
+            }
+            hipLaunchKernelGGL((k_br_fft<P, KB, 2>), g2, block, 0, st, jobs, njobs, mu, K);
             break;
+        }
         case 4:
             if constexpr (sizeof(typename P::T) == 4)
                 hipLaunchKernelGGL((k_br_fft<P, KB, 4>), grid, block, 0, st, jobs, njobs, mu,

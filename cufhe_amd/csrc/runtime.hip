@@ -3,6 +3,7 @@
 // batched plans (optionally captured in a HIP graph).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -545,18 +546,22 @@ bool ks_lvl1(const KsJob* jobs, int n, void* ws, hipStream_t st)
     return true;
 }
 
+// Jobs per workgroup of the lvl2 FFT rotation (1 or 2), cufhe_set_lvl2_jobs_per_group.
+std::atomic<int> g_lvl2_c{1};
+
 // Batched lvl2 blind rotation (+ SampleExtract or the TRLWE) on the loaded
 // lvl2 backend.
 void br_lvl2(const BrJob* jobs, int n, uint64_t mu, hipStream_t st)
 {
     const LevelKeys<lvl2param>& k = state().k2;
     if (n <= 0) return;
+    const int c = g_lvl2_c.load(std::memory_order_relaxed);
     if (k.backend == CUFHE_BACKEND_NTT)
         launch_br_ntt64(k.kb, jobs, n, mu, k.n64, st);
     else if (k.kb)
-        launch_br_fft<lvl2param, true>(jobs, n, 1, mu, k.fk, st);
+        launch_br_fft<lvl2param, true>(jobs, n, c, mu, k.fk, st);
     else
-        launch_br_fft<lvl2param, false>(jobs, n, 1, mu, k.fk, st);
+        launch_br_fft<lvl2param, false>(jobs, n, c, mu, k.fk, st);
 }
 
 // One lvl02 gate (BR-IKS) as a single launch on the loaded lvl2 backend.
@@ -1609,6 +1614,13 @@ int cufhe_autotune_lvl1(int nbootstraps, int* best, float* times_ms, void* strea
         t.best[n] = bestc;
     }
     if (best) *best = bestc;
+    return CUFHE_OK;
+}
+
+int cufhe_set_lvl2_jobs_per_group(int c)
+{
+    if (c != 1 && c != 2) return fail(CUFHE_ERR_ARG, "lvl2 jobs per workgroup must be 1 or 2");
+    g_lvl2_c.store(c, std::memory_order_relaxed);
     return CUFHE_OK;
 }
 

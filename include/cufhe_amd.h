@@ -206,6 +206,10 @@ int cufhe_autotune_lvl1(int nbootstraps, int* best, float* times_ms, void* strea
  * per-batch choice (about two waves per SIMD). Applies to launches made
  * after the call, and to graphs captured after it. */
 int cufhe_set_key_switch_split(int split);
+/* Tuning knob of the batched lvl2 FFT rotation, no reference counterpart:
+ * lvl2 jobs per workgroup (1 or 2; default 1). Applies to launches made
+ * after the call, and to graphs captured after it. */
+int cufhe_set_lvl2_jobs_per_group(int c);
 /* Convenience: plan + run + destroy (allocates per call). */
 int cufhe_gate_batch_lvl1(const cufhe_gate_desc* gates, int n, uint32_t* d_pool,
                           int jobs_per_group, void* stream);

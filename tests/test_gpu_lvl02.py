@@ -144,3 +144,25 @@ def test_lvl02_batch_larger_than_resident_grid(lvl02):
     ph = np.abs(C.phase(2, k2, big))
     print("lvl2 |phase| range", ph.min(), ph.max())
     assert np.all(np.abs(ph - 0.125) < 2 ** -16)
+
+
+def test_lvl02_two_jobs_per_workgroup_bit_identical(lvl02):
+    """The two-jobs-per-workgroup lvl2 rotation (cufhe_set_lvl2_jobs_per_group)
+    gives the one-job kernel's outputs bit for bit, including an odd count
+    (a dead slot in the last workgroup)."""
+    rng = np.random.default_rng(204)
+    n = 701
+    tl0 = rng.integers(0, 2**32, (n, W0), dtype=np.uint64).astype(np.uint32)
+    mu2 = 1 << 61
+    din = C.DeviceBuffer.from_numpy(tl0)
+    outs = []
+    try:
+        for c in (1, 2):
+            C.check(C.lib().cufhe_set_lvl2_jobs_per_group(c), "lvl2 jobs per group")
+            dout = C.DeviceBuffer(n * 2049 * 8)
+            C.check(C.lib().cufhe_bootstrap_lvl02(dout.ptr, din.ptr, mu2, n, None), "bootstrap")
+            outs.append(dout.download(np.uint64, (n, 2049)))
+    finally:
+        C.check(C.lib().cufhe_set_lvl2_jobs_per_group(1), "lvl2 jobs per group")
+    assert np.array_equal(outs[0], outs[1])
+    assert C.lib().cufhe_set_lvl2_jobs_per_group(3) != 0
