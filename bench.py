@@ -31,6 +31,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from cufhe_amd.noise import predicted_lvl1_variance  # noqa: E402  (host-side formula, no GPU)
+
 METRIC = "lvl1 NAND bootstraps/s @batch 4096 vs plain per-gate form; p50/p99 layer latency"
 FLOP_PER_BOOTSTRAP_KB = 91502592      # 318 x 287,744 (algorithmic, lvl1 KB=2)
 FLOP_PER_BOOTSTRAP_KB_LVL2 = 349077504  # 318 x 1,097,728 (lvl2 KB=2)
@@ -343,7 +345,10 @@ def workload_kb4096(args):
         "batched_over_plain": round(value / plain_tput, 2),
         "decrypt_failures": failures,
         "verified_chain": {"gates": verified, "layers": vlayers, "decrypt_failures": vfail,
-                           "output_noise": vnoise},
+                           "output_noise": vnoise,
+                           "predicted_variance": predicted_lvl1_variance("kb"),
+                           "predicted_from": "318 key-bundle steps, N=1024, l=2, Bg=2^8, "
+                                             "alpha_bk=2^-24.8, 6 key-noise terms per step (cufhe_amd/noise.py)"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                      "traffic": traffic,
@@ -531,7 +536,10 @@ def workload_nttbb(args):
         "batched_over_plain": round(value / plain_tput, 2),
         "decrypt_failures": failures,
         "verified_chain": {"gates": verified, "layers": vlayers, "decrypt_failures": vfail,
-                           "output_noise": vnoise},
+                           "output_noise": vnoise,
+                           "predicted_variance": predicted_lvl1_variance("bb"),
+                           "predicted_from": "159 block-binary blocks of 4, N=1024, l=2, Bg=2^8, "
+                                             "alpha_bk=2^-24.8, 8 key-noise terms per block (cufhe_amd/noise.py)"},
         "bit_exact_sample": bit_exact,
         # integer-only kernel bound by VALU issue: VALU wave-instructions per
         # launch (profiles/br_ntt_bb_lvl1_batch4096_counters.md, C=2) over

@@ -40,3 +40,11 @@ def test_cpp_launcher_layer():
     the reference's gate API calls them (src/cufhe_gates_gpu.cu:198-537)."""
     p = _run("test_launchers_gpu.bin")
     assert p.returncode == 0 and "ALL PASS" in p.stdout
+
+
+def test_cpp_many_stream_stress():
+    """1000 streams x 10 chained plain-form gates each, driven by polling
+    StreamQuery (reference test/test_intensive.cc): every Nand and Mux output
+    decrypts correctly."""
+    p = _run("test_intensive_gpu.bin")
+    assert p.returncode == 0 and "ALL PASS" in p.stdout
