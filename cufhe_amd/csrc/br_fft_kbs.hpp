@@ -46,7 +46,8 @@ struct KbsShared {
     static constexpr int N = lvl1param::n, H = N / 2, R = H / 64, n0 = lvl0param::n;
     cplx spec[2 * C][H];    // per wave: spectrum + transform scratch
     cplx ring[2][12][64];   // key chunks
-    cplx twl[R * 64];       // transform tables (copies of FFTTables)
+    cplx twl[64];       // transform tables (copies of FFTTables; row 0 of twl)
+    cplx tws[64];
     cplx tw64[64];
     cplx cr[R];
     cplx e2n_hi[2 * N / 64];  // exp(i pi 64 t / N)
@@ -109,7 +110,7 @@ __device__ __forceinline__ void blind_rotate_kbs(KbsShared<C>& sm, int slot, int
     const int wave = slot * 2 + j;
     cplx* const my_spec = sm.spec[wave];
     const cplx* const partner_spec = sm.spec[wave ^ 1];
-    const FFTTables tb{sm.twl, sm.tw64, sm.cr};
+    const FFTTables tb{sm.twl, sm.tw64, sm.cr, sm.tws};
 
     // test vector
     const uint32_t bar = sm.bar[slot];

@@ -47,7 +47,8 @@ template <int C>
 struct KbwShared {
     static constexpr int N = lvl1param::n, H = N / 2, R = H / 64, n0 = lvl0param::n;
     cplx ring[2][kKbwRpc<C>][12][64];  // key chunks: [buffer][register][slice][bin]
-    cplx twl[R * 64];         // transform tables (copies of FFTTables)
+    cplx twl[64];         // transform tables (copies of FFTTables; row 0 of twl)
+    cplx tws[64];
     cplx tw64[64];
     cplx cr[R];
     cplx e2n_hi[2 * N / 64];  // exp(i pi 64 t / N)
@@ -153,7 +154,7 @@ __device__ __forceinline__ void blind_rotate_kbw(KbwShared<C>& sm, int slot, uin
     auto tables = [&]() {
         int zo = 0;
         asm volatile("" : "+s"(zo));
-        return FFTTables{sm.twl + zo, sm.tw64 + zo, sm.cr + zo};
+        return FFTTables{sm.twl + zo, sm.tw64 + zo, sm.cr + zo, sm.tws + zo};
     };
 
     // test vector

@@ -207,7 +207,9 @@ __device__ __forceinline__ void wave_prio<true>(int p)
 
 // Tables of the folded transform (device memory, built at init), H = 64 R:
 //   twl[k2*64 + l] = exp(i*pi*l/N) * exp(-2*pi*i*l*k2/H)   (lane twist merged
-//                    with the step-2 twiddle), k2 < R, l < 64
+//                    with the step-2 twiddle), k2 < R, l < 64 (the transforms
+//                    read row 0 and build the others with tws, below)
+//   tws[l]         = exp(-2*pi*i*l/H)
 //   tw64[c*8 + a]  = exp(-2*pi*i*a*c/64)
 //   cr[r]          = exp(i*pi*64*r/N)                        (r < R)
 // twist[m] = exp(i*pi*m/N) factors as exp(i*pi*l/N) * cr[r] for m = l + 64 r;
@@ -216,7 +218,29 @@ struct FFTTables {
     const cplx* twl;
     const cplx* tw64;
     const cplx* cr;
+    const cplx* tws;  // tws[l] = exp(-2*pi*i*l/H)
 };
+
+// Twiddle sequences computed from one or two table entries by repeated
+// complex multiplication (R - 1 products instead of R - 1 table loads; the
+// loads, not the products, limit the transforms). Every transform builds them
+// in this one order, so all kernels -- and the key spectra -- see identical
+// twiddles:
+//   stage A: twl[k2*64 + l] = twl[l] * tws[l]^k2   (k2 = 0 .. R-1)
+//   stage B: tw64[c*8 + a]  = tw64[8 + a]^c        (c = 1 .. 7)
+struct TwiddleSeq {
+    cplx t, w;
+    __device__ __forceinline__ void next() { t = cmul(t, w); }
+};
+__device__ __forceinline__ TwiddleSeq stage_a_twiddles(const FFTTables& tb, int lane)
+{
+    return TwiddleSeq{tb.twl[lane], tb.tws[lane]};
+}
+__device__ __forceinline__ TwiddleSeq stage_b_twiddles(const FFTTables& tb, int lo)
+{
+    const cplx w = tb.tw64[8 + lo];
+    return TwiddleSeq{w, w};
+}
 
 // Stages of the transforms for the paired forms (fft_forward2 /
 // fft_inverse2): the operations of fft_forward / fft_inverse split at the
@@ -259,8 +283,12 @@ __device__ __forceinline__ void fwd_a(cplx (&x)[R], const FFTTables& tb, int lan
 #pragma unroll
     for (int r = 1; r < R; r++) x[r] = cmul(x[r], tb.cr[r]);
     dft_reg<R, false>(x);
+    TwiddleSeq ts = stage_a_twiddles(tb, lane);
 #pragma unroll
-    for (int k2 = 0; k2 < R; k2++) x[k2] = cmul(x[k2], tb.twl[k2 * 64 + lane]);
+    for (int k2 = 0; k2 < R; k2++) {
+        x[k2] = cmul(x[k2], ts.t);
+        if (k2 + 1 < R) ts.next();
+    }
 }
 // fwd_a of two transforms with each twiddle read once
 template <int R>
@@ -273,11 +301,12 @@ __device__ __forceinline__ void fwd_a2(cplx (&x)[R], cplx (&y)[R], const FFTTabl
     }
     dft_reg<R, false>(x);
     dft_reg<R, false>(y);
+    TwiddleSeq ts = stage_a_twiddles(tb, lane);
 #pragma unroll
     for (int k2 = 0; k2 < R; k2++) {
-        const cplx t = tb.twl[k2 * 64 + lane];
-        x[k2] = cmul(x[k2], t);
-        y[k2] = cmul(y[k2], t);
+        x[k2] = cmul(x[k2], ts.t);
+        y[k2] = cmul(y[k2], ts.t);
+        if (k2 + 1 < R) ts.next();
     }
 }
 template <int R, int LAY>
@@ -302,8 +331,12 @@ __device__ __forceinline__ void fwd_b(cplx (&x)[R], const FFTTables& tb, int lan
 #pragma unroll
     for (int g = 0; g < R / 8; g++) {
         dft_reg<8, false>(&x[g * 8]);
+        TwiddleSeq ts = stage_b_twiddles(tb, lo);
 #pragma unroll
-        for (int c = 1; c < 8; c++) x[g * 8 + c] = cmul(x[g * 8 + c], tb.tw64[c * 8 + lo]);
+        for (int c = 1; c < 8; c++) {
+            x[g * 8 + c] = cmul(x[g * 8 + c], ts.t);
+            if (c < 7) ts.next();
+        }
     }
 }
 template <int R, int LAY>
@@ -361,8 +394,12 @@ __device__ __forceinline__ void inv_b(cplx (&x)[R], const FFTTables& tb, int lan
 #pragma unroll
     for (int g = 0; g < R / 8; g++) {
         // undo W_64^(a*c): lane low bits are a here, register index c
+        TwiddleSeq ts = stage_b_twiddles(tb, lo);
 #pragma unroll
-        for (int c = 1; c < 8; c++) x[g * 8 + c] = cmulc(x[g * 8 + c], tb.tw64[c * 8 + lo]);
+        for (int c = 1; c < 8; c++) {
+            x[g * 8 + c] = cmulc(x[g * 8 + c], ts.t);
+            if (c < 7) ts.next();
+        }
         dft_reg<8, true>(&x[g * 8]);
     }
 }
@@ -384,8 +421,14 @@ __device__ __forceinline__ void t1_load_inv(cplx (&x)[R], const cplx* lds, int l
 template <int R>
 __device__ __forceinline__ void inv_c(cplx (&x)[R], const FFTTables& tb, int lane)
 {
+    {
+        TwiddleSeq ts = stage_a_twiddles(tb, lane);
 #pragma unroll
-    for (int k2 = 0; k2 < R; k2++) x[k2] = cmulc(x[k2], tb.twl[k2 * 64 + lane]);
+        for (int k2 = 0; k2 < R; k2++) {
+            x[k2] = cmulc(x[k2], ts.t);
+            if (k2 + 1 < R) ts.next();
+        }
+    }
     dft_reg<R, true>(x);
 #pragma unroll
     for (int r = 1; r < R; r++) x[r] = cmulc(x[r], tb.cr[r]);
@@ -394,11 +437,12 @@ __device__ __forceinline__ void inv_c(cplx (&x)[R], const FFTTables& tb, int lan
 template <int R>
 __device__ __forceinline__ void inv_c2(cplx (&x)[R], cplx (&y)[R], const FFTTables& tb, int lane)
 {
+    TwiddleSeq ts = stage_a_twiddles(tb, lane);
 #pragma unroll
     for (int k2 = 0; k2 < R; k2++) {
-        const cplx t = tb.twl[k2 * 64 + lane];
-        x[k2] = cmulc(x[k2], t);
-        y[k2] = cmulc(y[k2], t);
+        x[k2] = cmulc(x[k2], ts.t);
+        y[k2] = cmulc(y[k2], ts.t);
+        if (k2 + 1 < R) ts.next();
     }
     dft_reg<R, true>(x);
     dft_reg<R, true>(y);
@@ -425,8 +469,14 @@ __device__ __forceinline__ void fft_forward(cplx (&x)[H / 64], cplx* lds, const 
 #pragma unroll
     for (int r = 1; r < R; r++) x[r] = cmul(x[r], tb.cr[r]);
     dft_reg<R, false>(x);
+    {
+        TwiddleSeq ts = stage_a_twiddles(tb, lane);
 #pragma unroll
-    for (int k2 = 0; k2 < R; k2++) x[k2] = cmul(x[k2], tb.twl[k2 * 64 + lane]);
+        for (int k2 = 0; k2 < R; k2++) {
+            x[k2] = cmul(x[k2], ts.t);
+            if (k2 + 1 < R) ts.next();
+        }
+    }
 #pragma unroll
     for (int k2 = 0; k2 < R; k2++) lds_store(&lds[fft_t1<SWZ>(k2, lane)], x[k2]);
     wave_lds_sync();
@@ -440,8 +490,12 @@ __device__ __forceinline__ void fft_forward(cplx (&x)[H / 64], cplx* lds, const 
 #pragma unroll
     for (int g = 0; g < G; g++) {
         dft_reg<8, false>(&x[g * 8]);
+        TwiddleSeq ts = stage_b_twiddles(tb, lo);
 #pragma unroll
-        for (int c = 1; c < 8; c++) x[g * 8 + c] = cmul(x[g * 8 + c], tb.tw64[c * 8 + lo]);
+        for (int c = 1; c < 8; c++) {
+            x[g * 8 + c] = cmul(x[g * 8 + c], ts.t);
+            if (c < 7) ts.next();
+        }
 #pragma unroll
         for (int c = 0; c < 8; c++) lds_store(&lds[fft_t2<SWZ>(g * 8 + q, c, lo ^ c)], x[g * 8 + c]);
     }
@@ -483,8 +537,12 @@ __device__ __forceinline__ void fft_inverse(cplx (&x)[H / 64], cplx* lds, const 
 #pragma unroll
     for (int g = 0; g < G; g++) {
         // undo W_64^(a*c): lane low bits are a here, register index c
+        TwiddleSeq ts = stage_b_twiddles(tb, lo);
 #pragma unroll
-        for (int c = 1; c < 8; c++) x[g * 8 + c] = cmulc(x[g * 8 + c], tb.tw64[c * 8 + lo]);
+        for (int c = 1; c < 8; c++) {
+            x[g * 8 + c] = cmulc(x[g * 8 + c], ts.t);
+            if (c < 7) ts.next();
+        }
         dft_reg<8, true>(&x[g * 8]);
 #pragma unroll
         for (int b = 0; b < 8; b++) lds_store(&lds[fft_t1<SWZ>(g * 8 + q, 8 * b + lo)], x[g * 8 + b]);
@@ -494,8 +552,14 @@ __device__ __forceinline__ void fft_inverse(cplx (&x)[H / 64], cplx* lds, const 
     for (int k2 = 0; k2 < R; k2++) x[k2] = lds_load(&lds[fft_t1<SWZ>(k2, lane)]);
     wave_lds_sync();
     wave_prio<PRIO>(1);
+    {
+        TwiddleSeq ts = stage_a_twiddles(tb, lane);
 #pragma unroll
-    for (int k2 = 0; k2 < R; k2++) x[k2] = cmulc(x[k2], tb.twl[k2 * 64 + lane]);
+        for (int k2 = 0; k2 < R; k2++) {
+            x[k2] = cmulc(x[k2], ts.t);
+            if (k2 + 1 < R) ts.next();
+        }
+    }
     dft_reg<R, true>(x);
 #pragma unroll
     for (int r = 1; r < R; r++) x[r] = cmulc(x[r], tb.cr[r]);

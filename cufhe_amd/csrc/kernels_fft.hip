@@ -173,7 +173,10 @@ __global__ __launch_bounds__(128 * C, 1) void k_br_fft_kbs(const BrJob* __restri
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int slot = wave >> 1, j = wave & 1;
     const int t = threadIdx.x;
-    for (int i = t; i < R * 64; i += NT) sm.twl[i] = K.fft.twl[i];
+    for (int i = t; i < 64; i += NT) {
+        sm.twl[i] = K.fft.twl[i];
+        sm.tws[i] = K.fft.tws[i];
+    }
     for (int i = t; i < 64; i += NT) {
         sm.tw64[i] = K.fft.tw64[i];
         sm.e2n_lo[i] = K.e2n[i];
@@ -221,7 +224,10 @@ __global__ __launch_bounds__(64 * C, 8 / C) void k_br_fft_kbw(const BrJob* __res
     __shared__ KbwShared<C> sm;
     const int slot = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int t = threadIdx.x;
-    for (int i = t; i < R * 64; i += NT) sm.twl[i] = K.fft.twl[i];
+    for (int i = t; i < 64; i += NT) {
+        sm.twl[i] = K.fft.twl[i];
+        sm.tws[i] = K.fft.tws[i];
+    }
     for (int i = t; i < 64; i += NT) {
         sm.tw64[i] = K.fft.tw64[i];
         sm.e2n_lo[i] = K.e2n[i];
@@ -428,7 +434,10 @@ __global__ __launch_bounds__(512, 1) void k_gate_lvl1_kbw(
     __shared__ uint32_t sh_res[MUX ? 2 : 1][MUX ? N + 1 : 1];
     const int slot = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int t = threadIdx.x;
-    for (int i = t; i < R * 64; i += NT) sm.twl[i] = K.fft.twl[i];
+    for (int i = t; i < 64; i += NT) {
+        sm.twl[i] = K.fft.twl[i];
+        sm.tws[i] = K.fft.tws[i];
+    }
     for (int i = t; i < 64; i += NT) {
         sm.tw64[i] = K.fft.tw64[i];
         sm.e2n_lo[i] = K.e2n[i];
@@ -506,7 +515,10 @@ __global__ __launch_bounds__(512, 1) void k_gate_lvl01_kbw(
     __shared__ uint32_t sh_res[NJ][N + 4];
     const int slot = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int t = threadIdx.x;
-    for (int i = t; i < R * 64; i += NT) sm.twl[i] = K.fft.twl[i];
+    for (int i = t; i < 64; i += NT) {
+        sm.twl[i] = K.fft.twl[i];
+        sm.tws[i] = K.fft.tws[i];
+    }
     for (int i = t; i < 64; i += NT) {
         sm.tw64[i] = K.fft.tw64[i];
         sm.e2n_lo[i] = K.e2n[i];

@@ -47,6 +47,7 @@ struct TableSet {
     cplx* twl = nullptr;
     cplx* tw64 = nullptr;
     cplx* cr = nullptr;
+    cplx* tws = nullptr;
     cplx* e2n = nullptr;
 };
 
@@ -131,13 +132,15 @@ int build_tables(TableSet<P>& ts)
     constexpr int N = P::n, H = N / 2, R = H / 64;
     const std::vector<cplx> E = e2n_table(N);
     auto at = [&](long long t) { return E[static_cast<size_t>(((t % (2 * N)) + 2 * N) % (2 * N))]; };
-    std::vector<cplx> twl(R * 64), tw64(64), cr(R);
+    std::vector<cplx> twl(R * 64), tw64(64), cr(R), tws(64);
     // twl[k2*64+l] = exp(i*pi*l/N) * exp(-2*pi*i*l*k2/H) = E[l*(1 - 4*k2)]
     for (int k2 = 0; k2 < R; k2++)
         for (int l = 0; l < 64; l++) twl[k2 * 64 + l] = at(static_cast<long long>(l) * (1 - 4 * k2));
     // tw64[c*8+a] = exp(-2*pi*i*a*c/64) = E[-(2N/64)*a*c]
     for (int c = 0; c < 8; c++)
         for (int a = 0; a < 8; a++) tw64[c * 8 + a] = at(-static_cast<long long>(2 * N / 64) * a * c);
+    // tws[l] = exp(-2*pi*i*l/H) = E[-4 l]
+    for (int l = 0; l < 64; l++) tws[l] = at(-4LL * l);
     // cr[r] = exp(i*pi*64*r/N) = E[64 r]
     for (int r = 0; r < R; r++) cr[r] = at(64LL * r);
     auto up = [](cplx** dst, const std::vector<cplx>& v) -> hipError_t {
@@ -148,6 +151,7 @@ int build_tables(TableSet<P>& ts)
     CUFHE_HIP_TRY(up(&ts.twl, twl));
     CUFHE_HIP_TRY(up(&ts.tw64, tw64));
     CUFHE_HIP_TRY(up(&ts.cr, cr));
+    CUFHE_HIP_TRY(up(&ts.tws, tws));
     CUFHE_HIP_TRY(up(&ts.e2n, E));
     return CUFHE_OK;
 }
@@ -261,6 +265,7 @@ FourierKeys<P> keys_view(const TableSet<P>& ts, const cplx* bk)
     k.fft.twl = ts.twl;
     k.fft.tw64 = ts.tw64;
     k.fft.cr = ts.cr;
+    k.fft.tws = ts.tws;
     return k;
 }
 
@@ -270,6 +275,7 @@ void free_tables(TableSet<P>& ts)
     (void)hipFree(ts.twl);
     (void)hipFree(ts.tw64);
     (void)hipFree(ts.cr);
+    (void)hipFree(ts.tws);
     (void)hipFree(ts.e2n);
     ts = TableSet<P>{};
 }
