@@ -67,11 +67,19 @@ inline constexpr uint32_t kMaskRound = 1u << (32 - 2 - TP::nbit);
 template <class T>
 __device__ __forceinline__ T double_to_torus(double v)
 {
-    const double r = rint(v);
     if constexpr (sizeof(T) == 4) {
-        return static_cast<T>(static_cast<int64_t>(r));
+        // rint(v) mod 2^32 in four instructions: q = rint(v / 2^32) and
+        // r = v - q 2^32 (exact: |r| <= 2^31, a multiple of v's ulp), then
+        // the low word of r + 1.5 * 2^52 is rint(r) mod 2^32 (round to nearest
+        // even at the unit position; |r| < 2^51). rint(v) = q 2^32 + rint(r)
+        // with q 2^32 even, so the result equals (uint32_t)(int64_t)rint(v).
+        const double q = rint(v * (1.0 / 4294967296.0));
+        const double r = fma(-q, 4294967296.0, v);
+        const double t = r + 6755399441055744.0;
+        return static_cast<T>(static_cast<uint64_t>(__double_as_longlong(t)));
     }
     else {
+        const double r = rint(v);
         if (fabs(r) < 9223372036854775808.0) return static_cast<T>(static_cast<int64_t>(r));
         const uint64_t bits = static_cast<uint64_t>(__double_as_longlong(r));
         const int ex = static_cast<int>((bits >> 52) & 0x7FF) - 1075;
