@@ -224,22 +224,11 @@ __device__ __forceinline__ void br_step_fft(AccStore<P>& acc, int j, uint32_t a0
 
 #pragma unroll 1
     for (int ps = 0; ps < S::kPasses; ps++) {
-        if constexpr (!GHOST) {
-            const int qd = ps * NB;
-            cplx z[R];
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                z[r] = cplx{static_cast<double>(Gadget<P>::digit(src_at(r), qd)),
-                            static_cast<double>(Gadget<P>::digit(src_at(R + r), qd))};
-            }
-            fft_forward<H>(z, my_spec, tb);
-#pragma unroll
-            for (int d = 0; d < R; d++) lds_store(&my_spec[d * 64 + lane], z[d]);
-        }
         // Key loads of this pass: column j of rows (comp * L + q), comp = 0, 1.
         // They do not depend on the spectra, so the first PD registers' worth
-        // is issued before the pair barrier and the rest PD registers ahead
-        // (L2 latency is hidden by loads in flight, not by occupancy).
+        // is issued before the pass's forward transform (which hides their
+        // latency) and the rest PD registers ahead (L2 latency is hidden by
+        // loads in flight, not by occupancy).
         struct KeyVals {
             cplx k[2][KB ? 3 : 1];
         };
@@ -262,6 +251,18 @@ __device__ __forceinline__ void br_step_fft(AccStore<P>& acc, int j, uint32_t a0
         if constexpr (!GHOST) {
 #pragma unroll
             for (int d = 0; d < PD; d++) kv[d] = load_keys(d);
+        }
+        if constexpr (!GHOST) {
+            const int qd = ps * NB;
+            cplx z[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                z[r] = cplx{static_cast<double>(Gadget<P>::digit(src_at(r), qd)),
+                            static_cast<double>(Gadget<P>::digit(src_at(R + r), qd))};
+            }
+            fft_forward<H>(z, my_spec, tb);
+#pragma unroll
+            for (int d = 0; d < R; d++) lds_store(&my_spec[d * 64 + lane], z[d]);
         }
         __syncthreads();
         if constexpr (KB && !GHOST) {
