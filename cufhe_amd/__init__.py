@@ -150,6 +150,9 @@ def lib():
         "tfhe_tlwe_decrypt_bits": ([i32, vp, vp, sz, vp], i32),
         "tfhe_tlwe_words": ([i32], sz),
         "tfhe_bk_words": ([i32, i32], sz),
+        "tfhe_keyfile_save": ([ctypes.c_char_p, u32, vp, vp], i32),
+        "tfhe_keyfile_info": ([ctypes.c_char_p, vp, vp], i32),
+        "tfhe_keyfile_load": ([ctypes.c_char_p, vp], i32),
         "tfhe_ksk_words": ([i32], sz),
         "tfhe_ksk_subset_words": ([], sz),
     }
@@ -181,6 +184,47 @@ def secret_key_gen(seed, block_binary=False):
     check(lib().tfhe_secret_key_gen(seed, int(block_binary), _ptr(k0), _ptr(k1), _ptr(k2)),
           "tfhe_secret_key_gen")
     return k0, k1, k2
+
+
+# Key files (tfhe_keyfile_*): record names in tag order and element types.
+KEYFILE_RECORDS = [("bklvl01", np.uint32), ("bklvl02", np.uint64), ("iksklvl10", np.uint32),
+                   ("iksklvl20", np.uint32), ("sklvl0", np.uint32), ("sklvl1", np.uint32),
+                   ("sklvl2", np.uint64)]
+KEYFILE_FLAGS = [("key_bundle", 1), ("block_binary", 2), ("subset", 4), ("ntt", 8)]
+
+
+def save_keyfile(path, arrays, key_bundle=True, block_binary=False, subset=False, ntt=False):
+    """Write TFHEpp-layout key arrays (names of KEYFILE_RECORDS) to a key file."""
+    flags = sum(bit for name, bit in KEYFILE_FLAGS
+                if {"key_bundle": key_bundle, "block_binary": block_binary, "subset": subset,
+                    "ntt": ntt}[name])
+    known = dict(KEYFILE_RECORDS)
+    unknown = set(arrays) - set(known)
+    if unknown:
+        raise CufheError("unknown key records: %s" % sorted(unknown))
+    keep = [np.ascontiguousarray(arrays[n], known[n]) if n in arrays else None
+            for n, _ in KEYFILE_RECORDS]
+    ptrs = (ctypes.c_void_p * len(keep))(*[None if a is None else a.ctypes.data for a in keep])
+    counts = (ctypes.c_size_t * len(keep))(*[0 if a is None else a.size for a in keep])
+    rc = lib().tfhe_keyfile_save(os.fsencode(path), flags, ptrs, counts)
+    if rc != 0:
+        raise CufheError("tfhe_keyfile_save failed (%d): bad sizes or I/O" % rc)
+
+
+def load_keyfile(path):
+    """Read a key file: (flags dict, {record name: numpy array})."""
+    flags = ctypes.c_uint32(0)
+    counts = (ctypes.c_size_t * len(KEYFILE_RECORDS))()
+    rc = lib().tfhe_keyfile_info(os.fsencode(path), ctypes.byref(flags), counts)
+    if rc != 0:
+        raise CufheError("tfhe_keyfile_info failed (%d)" % rc)
+    arrays = {n: np.zeros(counts[i], dt) for i, (n, dt) in enumerate(KEYFILE_RECORDS) if counts[i]}
+    ptrs = (ctypes.c_void_p * len(KEYFILE_RECORDS))(
+        *[arrays[n].ctypes.data if n in arrays else None for n, _ in KEYFILE_RECORDS])
+    rc = lib().tfhe_keyfile_load(os.fsencode(path), ptrs)
+    if rc != 0:
+        raise CufheError("tfhe_keyfile_load failed (%d)" % rc)
+    return {name: bool(flags.value & bit) for name, bit in KEYFILE_FLAGS}, arrays
 
 
 def bk_gen(level, key_bundle, seed, k0, kt):

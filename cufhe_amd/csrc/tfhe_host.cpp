@@ -8,6 +8,7 @@
 // results do not depend on thread count.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -461,6 +462,119 @@ size_t tfhe_ksk_words(int domain_level)
     if (domain_level == 1) return kKSKWords<lvl10param>;
     if (domain_level == 2) return kKSKWords<lvl20param>;
     return 0;
+}
+
+namespace {
+constexpr char kKeyMagic[8] = {'C', 'U', 'F', 'H', 'E', 'K', 'Y', '1'};
+
+// Element size and expected element count of record tag t under `flags`.
+bool keyfile_shape(int tag, uint32_t flags, uint32_t* esize, size_t* count)
+{
+    const int kb = (flags & 1u) ? 1 : 0;
+    switch (tag) {
+        case TFHE_KEYFILE_BKLVL01: *esize = 4; *count = tfhe_bk_words(1, kb); return true;
+        case TFHE_KEYFILE_BKLVL02: *esize = 8; *count = tfhe_bk_words(2, kb); return true;
+        case TFHE_KEYFILE_IKSKLVL10:
+            *esize = 4;
+            *count = (flags & 4u) ? tfhe_ksk_subset_words() : tfhe_ksk_words(1);
+            return true;
+        case TFHE_KEYFILE_IKSKLVL20: *esize = 4; *count = tfhe_ksk_words(2); return true;
+        case TFHE_KEYFILE_SKLVL0: *esize = 4; *count = lvl0param::n; return true;
+        case TFHE_KEYFILE_SKLVL1: *esize = 4; *count = lvl1param::n; return true;
+        case TFHE_KEYFILE_SKLVL2: *esize = 8; *count = lvl2param::n; return true;
+        default: return false;
+    }
+}
+
+struct FileCloser {
+    FILE* f;
+    ~FileCloser()
+    {
+        if (f) std::fclose(f);
+    }
+};
+}  // namespace
+
+int tfhe_keyfile_save(const char* path, uint32_t flags, const void* const* ptrs,
+                      const size_t* counts)
+{
+    if (!path || !ptrs || !counts || (flags >> 4) != 0) return -1;
+    uint32_t nrec = 0;
+    for (int t = 1; t <= TFHE_KEYFILE_RECORDS; t++) {
+        if (!ptrs[t - 1] || !counts[t - 1]) continue;
+        uint32_t es = 0;
+        size_t want = 0;
+        if (!keyfile_shape(t, flags, &es, &want) || counts[t - 1] != want) return -1;
+        nrec++;
+    }
+    FileCloser fc{std::fopen(path, "wb")};
+    if (!fc.f) return -2;
+    bool ok = std::fwrite(kKeyMagic, 1, 8, fc.f) == 8 && std::fwrite(&flags, 4, 1, fc.f) == 1 &&
+              std::fwrite(&nrec, 4, 1, fc.f) == 1;
+    for (int t = 1; ok && t <= TFHE_KEYFILE_RECORDS; t++) {
+        if (!ptrs[t - 1] || !counts[t - 1]) continue;
+        uint32_t es = 0;
+        size_t want = 0;
+        keyfile_shape(t, flags, &es, &want);
+        const uint32_t tag = static_cast<uint32_t>(t);
+        const uint64_t n = counts[t - 1];
+        ok = std::fwrite(&tag, 4, 1, fc.f) == 1 && std::fwrite(&es, 4, 1, fc.f) == 1 &&
+             std::fwrite(&n, 8, 1, fc.f) == 1 && std::fwrite(ptrs[t - 1], es, n, fc.f) == n;
+    }
+    if (!ok) return -2;
+    return std::fflush(fc.f) == 0 ? 0 : -2;
+}
+
+// Walk the records; with ptrs, copy each present array into ptrs[tag-1].
+static int keyfile_walk(const char* path, uint32_t* flags_out, size_t* counts, void* const* ptrs)
+{
+    if (!path) return -1;
+    FileCloser fc{std::fopen(path, "rb")};
+    if (!fc.f) return -2;
+    char magic[8];
+    uint32_t flags = 0, nrec = 0;
+    if (std::fread(magic, 1, 8, fc.f) != 8 || std::memcmp(magic, kKeyMagic, 8) != 0) return -1;
+    if (std::fread(&flags, 4, 1, fc.f) != 1 || std::fread(&nrec, 4, 1, fc.f) != 1) return -1;
+    if ((flags >> 4) != 0 || nrec > TFHE_KEYFILE_RECORDS) return -1;
+    if (counts)
+        for (int t = 0; t < TFHE_KEYFILE_RECORDS; t++) counts[t] = 0;
+    bool seen[TFHE_KEYFILE_RECORDS] = {};
+    for (uint32_t i = 0; i < nrec; i++) {
+        uint32_t tag = 0, es = 0;
+        uint64_t n = 0;
+        if (std::fread(&tag, 4, 1, fc.f) != 1 || std::fread(&es, 4, 1, fc.f) != 1 ||
+            std::fread(&n, 8, 1, fc.f) != 1)
+            return -1;
+        uint32_t want_es = 0;
+        size_t want_n = 0;
+        if (tag < 1 || tag > TFHE_KEYFILE_RECORDS || seen[tag - 1] ||
+            !keyfile_shape(static_cast<int>(tag), flags, &want_es, &want_n) || es != want_es ||
+            n != want_n)
+            return -1;
+        seen[tag - 1] = true;
+        if (counts) counts[tag - 1] = static_cast<size_t>(n);
+        void* dst = ptrs ? ptrs[tag - 1] : nullptr;
+        if (dst) {
+            if (std::fread(dst, es, n, fc.f) != n) return -2;
+        }
+        else if (std::fseek(fc.f, static_cast<long>(es * n), SEEK_CUR) != 0) {
+            return -2;
+        }
+    }
+    if (flags_out) *flags_out = flags;
+    return 0;
+}
+
+int tfhe_keyfile_info(const char* path, uint32_t* flags, size_t* counts)
+{
+    if (!flags || !counts) return -1;
+    return keyfile_walk(path, flags, counts, nullptr);
+}
+
+int tfhe_keyfile_load(const char* path, void* const* ptrs)
+{
+    if (!ptrs) return -1;
+    return keyfile_walk(path, nullptr, nullptr, ptrs);
 }
 
 }  // extern "C"

@@ -79,6 +79,37 @@ int tfhe_tlwe_phase(int level, const void* key, const void* ct, size_t count,
 int tfhe_tlwe_decrypt_bits(int level, const void* key, const void* ct,
                            size_t count, uint8_t* bits_out);
 
+/* ---- key files (no reference counterpart: TFHEpp serialises its EvalKey
+ * with cereal, reference src/CMakeLists.txt:34; the un-vendored TFHEpp's
+ * archive layout is not available, so this is a documented container of the
+ * same TFHEpp-layout arrays). File: "CUFHEKY1", uint32 flags, uint32 record
+ * count, then per record uint32 tag, uint32 element bytes (4 or 8), uint64
+ * element count and the elements, little-endian. Tags: 1 bklvl01 (u32),
+ * 2 bklvl02 (u64), 3 iksklvl10 (u32, full or subset), 4 iksklvl20 (u32),
+ * 5 lvl0 secret key (u32), 6 lvl1 secret key (u32), 7 lvl2 secret key (u64).
+ * Flags: bit 0 key bundle, bit 1 block-binary, bit 2 subset iksklvl10,
+ * bit 3 NTT backend. Arrays absent from a file are passed as null / 0. */
+#define TFHE_KEYFILE_BKLVL01 1
+#define TFHE_KEYFILE_BKLVL02 2
+#define TFHE_KEYFILE_IKSKLVL10 3
+#define TFHE_KEYFILE_IKSKLVL20 4
+#define TFHE_KEYFILE_SKLVL0 5
+#define TFHE_KEYFILE_SKLVL1 6
+#define TFHE_KEYFILE_SKLVL2 7
+#define TFHE_KEYFILE_RECORDS 7
+/* Write the arrays ptrs[t-1] of counts[t-1] elements (t = tag; null or 0 =
+ * absent). Sizes are checked against the layouts: bootstrapping keys
+ * tfhe_bk_words(level, flags & 1), key-switching keys tfhe_ksk_words (or
+ * tfhe_ksk_subset_words with flags bit 2), secret keys n of their level.
+ * Returns 0, or -1 (bad arguments / sizes) or -2 (I/O). */
+int tfhe_keyfile_save(const char* path, uint32_t flags, const void* const* ptrs,
+                      const size_t* counts);
+/* Read a key file's flags and per-tag element counts (0 = absent). */
+int tfhe_keyfile_info(const char* path, uint32_t* flags, size_t* counts);
+/* Read the arrays into ptrs[t-1] (null: skip), each sized as reported by
+ * tfhe_keyfile_info. Returns 0, -1 (format) or -2 (I/O). */
+int tfhe_keyfile_load(const char* path, void* const* ptrs);
+
 /* Words of the structures above (for sizing buffers). */
 size_t tfhe_tlwe_words(int level);
 size_t tfhe_bk_words(int target_level, int key_bundle);

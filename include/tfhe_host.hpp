@@ -68,10 +68,90 @@ struct SecretKey {
                       "tfhe_secret_key_gen");
     }
     SecretKey() : SecretKey(std::random_device{}() | (static_cast<uint64_t>(std::random_device{}()) << 32)) {}
+    // Key files (tfhe_host.h): the three keys and the block-binary flag.
+    bool save(const char* path) const
+    {
+        const void* ptrs[TFHE_KEYFILE_RECORDS] = {};
+        size_t counts[TFHE_KEYFILE_RECORDS] = {};
+        ptrs[TFHE_KEYFILE_SKLVL0 - 1] = key.lvl0.data();
+        counts[TFHE_KEYFILE_SKLVL0 - 1] = key.lvl0.size();
+        ptrs[TFHE_KEYFILE_SKLVL1 - 1] = key.lvl1.data();
+        counts[TFHE_KEYFILE_SKLVL1 - 1] = key.lvl1.size();
+        ptrs[TFHE_KEYFILE_SKLVL2 - 1] = key.lvl2.data();
+        counts[TFHE_KEYFILE_SKLVL2 - 1] = key.lvl2.size();
+        return tfhe_keyfile_save(path, block_binary ? 2u : 0u, ptrs, counts) == 0;
+    }
+    bool load(const char* path)
+    {
+        uint32_t fl = 0;
+        size_t counts[TFHE_KEYFILE_RECORDS] = {};
+        if (tfhe_keyfile_info(path, &fl, counts) != 0) return false;
+        if (!counts[TFHE_KEYFILE_SKLVL0 - 1] || !counts[TFHE_KEYFILE_SKLVL1 - 1] ||
+            !counts[TFHE_KEYFILE_SKLVL2 - 1])
+            return false;
+        void* ptrs[TFHE_KEYFILE_RECORDS] = {};
+        ptrs[TFHE_KEYFILE_SKLVL0 - 1] = key.lvl0.data();
+        ptrs[TFHE_KEYFILE_SKLVL1 - 1] = key.lvl1.data();
+        ptrs[TFHE_KEYFILE_SKLVL2 - 1] = key.lvl2.data();
+        if (tfhe_keyfile_load(path, ptrs) != 0) return false;
+        block_binary = (fl & 2u) != 0;
+        return true;
+    }
 };
 
 // Bootstrapping keys (polynomial domain) and key-switching keys.
 struct EvalKey {
+    // Key files (tfhe_keyfile_save/_load, tfhe_host.h): the present arrays
+    // and the flags (key bundle, block-binary, subset iksklvl10, NTT).
+    // Returns false on an I/O or format error.
+    bool save(const char* path) const
+    {
+        const void* ptrs[TFHE_KEYFILE_RECORDS] = {};
+        size_t counts[TFHE_KEYFILE_RECORDS] = {};
+        auto put = [&](int tag, const auto& v) {
+            if (v) {
+                ptrs[tag - 1] = v->data();
+                counts[tag - 1] = v->size();
+            }
+        };
+        put(TFHE_KEYFILE_BKLVL01, bklvl01);
+        put(TFHE_KEYFILE_BKLVL02, bklvl02);
+        put(TFHE_KEYFILE_IKSKLVL10, iksklvl10);
+        put(TFHE_KEYFILE_IKSKLVL20, iksklvl20);
+        return tfhe_keyfile_save(path, flags(), ptrs, counts) == 0;
+    }
+    bool load(const char* path)
+    {
+        uint32_t fl = 0;
+        size_t counts[TFHE_KEYFILE_RECORDS] = {};
+        if (tfhe_keyfile_info(path, &fl, counts) != 0) return false;
+        void* ptrs[TFHE_KEYFILE_RECORDS] = {};
+        auto take = [&](int tag, auto& v) {
+            using V = typename std::remove_reference_t<decltype(v)>::element_type;
+            if (counts[tag - 1]) {
+                v = std::make_unique<V>(counts[tag - 1]);
+                ptrs[tag - 1] = v->data();
+            }
+            else {
+                v.reset();
+            }
+        };
+        take(TFHE_KEYFILE_BKLVL01, bklvl01);
+        take(TFHE_KEYFILE_BKLVL02, bklvl02);
+        take(TFHE_KEYFILE_IKSKLVL10, iksklvl10);
+        take(TFHE_KEYFILE_IKSKLVL20, iksklvl20);
+        if (tfhe_keyfile_load(path, ptrs) != 0) return false;
+        key_bundle = (fl & 1u) != 0;
+        block_binary = (fl & 2u) != 0;
+        subset_iksk = (fl & 4u) != 0;
+        ntt = (fl & 8u) != 0;
+        return true;
+    }
+    uint32_t flags() const
+    {
+        return (key_bundle ? 1u : 0u) | (block_binary ? 2u : 0u) | (subset_iksk ? 4u : 0u) |
+               (ntt ? 8u : 0u);
+    }
     bool key_bundle = true;    // 2-bit key bundle (reference default build)
     bool block_binary = false;  // block-binary rotation (NTT backend; reference USE_BLOCK_BINARY)
     bool subset_iksk = false;   // iksklvl10 holds the subset key (emplacesubiksk)
