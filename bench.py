@@ -36,8 +36,18 @@ sys.path.insert(0, ROOT)
 # 4), so at most 4 such gates run at once. 16 queues measured 3.4x the plain
 # form's throughput (987 -> 3,320 gates/s; 32 measured 1,637); the batched
 # path uses one stream and is unaffected. Set before HIP initialises (no HIP
-# call happens before main); an explicit setting in the environment wins.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# call happens before main); --hw-queues 0 keeps the environment's value.
+def _hw_queues(argv):
+    for i, a in enumerate(argv):
+        if a == "--hw-queues" and i + 1 < len(argv):
+            return argv[i + 1]
+        if a.startswith("--hw-queues="):
+            return a.split("=", 1)[1]
+    return "16"
+
+
+if _hw_queues(sys.argv) != "0":
+    os.environ["GPU_MAX_HW_QUEUES"] = _hw_queues(sys.argv)
 
 from cufhe_amd.noise import predicted_lvl1_variance  # noqa: E402  (host-side formula, no GPU)
 
@@ -349,6 +359,7 @@ def workload_kb4096(args):
         "kernel_ms": {"key_switch": round(statistics.mean(ks_ms), 3),
                       "blind_rotate": round(statistics.mean(br_ms), 3)},
         "plain_form": {"gates_per_s": round(plain_tput, 2), "streams": args.plain_streams,
+                       "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
                        "sample": P, "latency_p50_ms": round(plain_lat, 3)},
         "batched_over_plain": round(value / plain_tput, 2),
         "decrypt_failures": failures,
@@ -458,7 +469,8 @@ def workload_lvl2mix(args):
         "per_type": {"mix": mix, "binary": binr, "mux": muxr},
         "plain_form": {"binary_gates_per_s": round(pb, 2), "mux_gates_per_s": round(pm, 2),
                        "binary_latency_p50_ms": round(pb_lat, 3),
-                       "mux_latency_p50_ms": round(pm_lat, 3), "streams": args.plain_streams},
+                       "mux_latency_p50_ms": round(pm_lat, 3), "streams": args.plain_streams,
+                       "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES")},
         "batched_over_plain": {"binary": round(binr["gates_per_s"] / pb, 2),
                                "mux": round(muxr["gates_per_s"] / pm, 2)},
         "roofline": roof,
@@ -540,6 +552,7 @@ def workload_nttbb(args):
         "kernel_ms": {"key_switch": round(statistics.mean(ks_ms), 3),
                       "blind_rotate": round(statistics.mean(br_ms), 3)},
         "plain_form": {"gates_per_s": round(plain_tput, 2), "streams": args.plain_streams,
+                       "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
                        "latency_p50_ms": round(plain_lat, 3)},
         "batched_over_plain": round(value / plain_tput, 2),
         "decrypt_failures": failures,
@@ -674,6 +687,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="kb4096")
     ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="GPU_MAX_HW_QUEUES for this process (1-32; 0 keeps the environment's)")
     ap.add_argument("--lvl2-backend", choices=["fft", "ntt"], default="fft",
                     help="lvl2mix: blind rotation on the FP64 FFT or the Goldilocks NTT backend")
     ap.add_argument("--lvl2-jobs-per-group", type=int, default=0,
