@@ -47,7 +47,7 @@ def _hw_queues(argv):
 
 
 if _hw_queues(sys.argv) != "0":
-    os.environ["GPU_MAX_HW_QUEUES"] = _hw_queues(sys.argv)
+    os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, max(1, int(_hw_queues(sys.argv)))))
 
 from cufhe_amd.noise import predicted_lvl1_variance  # noqa: E402  (host-side formula, no GPU)
 
