@@ -193,3 +193,38 @@ def test_lvl02_gates_bit_exact_vs_oracle(engine, keys_lvl2):
     # restore the FFT backend for the other lvl02 tests
     C.initialize_lvl02(keys_lvl2["bk_kb"], keys_lvl2["ksk"], backend=C.BACKEND_FFT,
                        key_bundle=True)
+
+
+def test_lvl02_mix_full_size_on_ntt_backend(engine, keys_lvl2):
+    """The lvl02 benchmark mix at its full size (1536 binary gates uniform
+    over NAND/AND/OR/XOR/XNOR/NOR + 512 MUX, interleaved) as one batched plan
+    on the Goldilocks backend: every output decrypts to the plaintext
+    evaluation, and a sample of gates equals the oracle's NTT mode word for
+    word."""
+    k0 = keys_lvl2["k0"]
+    C.initialize_lvl02(keys_lvl2["bk_kb"], keys_lvl2["ksk"], backend=C.BACKEND_NTT,
+                       key_bundle=True)
+    rng = np.random.default_rng(77)
+    mix = ["NAND", "AND", "OR", "XOR", "XNOR", "NOR"]
+    ops = [mix[i] for i in rng.integers(0, len(mix), 1536)] + ["MUX"] * 512
+    ops = [ops[i] for i in rng.permutation(len(ops))]
+    n = len(ops)
+    bits = rng.integers(0, 2, 3 * n).astype(np.uint8)
+    pool = np.zeros((4 * n, W0), np.uint32)
+    pool[:3 * n] = C.encrypt_bits(0, 4100, k0, bits)
+    dpool = C.DeviceBuffer.from_numpy(pool)
+    plan = C.Plan([(ops[i], 3 * n + i, i, n + i, 2 * n + i) for i in range(n)], dpool, level=2)
+    plan.run()
+    C.check(C.lib().cufhe_synchronize(), "sync")
+    plan.destroy()
+    out = dpool.download(np.uint32, pool.shape)[3 * n:]
+    x, y, s = bits[:n], bits[n:2 * n], bits[2 * n:]
+    want = np.array([(y[i] if s[i] else x[i]) if o == "MUX" else C.TRUTH[o](int(x[i]), int(y[i]))
+                     for i, o in enumerate(ops)], np.uint8)
+    assert np.array_equal(C.decrypt_bits(0, k0, out), want)
+    for i in (0, n // 2, n - 1):
+        e = O.gate_lvl02(O.NTT, True, C.GATES[ops[i]], pool[i], pool[n + i], pool[2 * n + i],
+                         keys_lvl2["bk_kb"], keys_lvl2["ksk"])
+        assert np.array_equal(e, out[i]), (i, ops[i])
+    C.initialize_lvl02(keys_lvl2["bk_kb"], keys_lvl2["ksk"], backend=C.BACKEND_FFT,
+                       key_bundle=True)
