@@ -135,6 +135,14 @@ def lib():
         "cufhe_cmux_batch_lvl1": ([vp, i32, vp], i32),
         "tfhe_trgsw_encrypt": ([u64, u64, vp, i32, vp], i32),
         "tfhe_trlwe_encrypt": ([u64, u64, vp, vp, ctypes.c_double, vp], i32),
+        "tfhe_trlwe_encrypt_level": ([i32, u64, u64, vp, vp, ctypes.c_double, vp], i32),
+        "tfhe_trlwe_phase": ([i32, vp, vp, sz, vp], i32),
+        "tfhe_annihilate_key_words": ([i32], sz),
+        "tfhe_annihilate_key_gen": ([i32, u64, vp, ctypes.c_double, vp, sz], i32),
+        "cufhe_annihilate_key_load": ([i32, vp, sz], i32),
+        "cufhe_annihilate_key_delete": ([i32], i32),
+        "cufhe_annihilate_key_switch": ([i32, vp, vp, vp], i32),
+        "cufhe_annihilate_key_switch_batch": ([i32, vp, sz, vp, sz, sz, vp], i32),
         "cufhe_server_create": ([sz, i32, i32, ctypes.POINTER(vp)], i32),
         "cufhe_server_submit": ([vp, vp, vp, vp, ctypes.POINTER(u64)], i32),
         "cufhe_server_wait": ([vp, u64, i32], i32),
@@ -308,6 +316,52 @@ def trlwe_encrypt(seed, key1, polys, alpha=3.42338787018369e-08):
         check(lib().tfhe_trlwe_encrypt(seed, i, _ptr(key1), _ptr(polys[i]), alpha, _ptr(out[i])),
               "tfhe_trlwe_encrypt")
     return out
+
+
+ALPHA = {1: 3.42338787018369e-08, 2: 5.68434188608080e-14}  # the levels' TRLWE noise
+
+
+def _torus(level):
+    return np.uint32 if level == 1 else np.uint64
+
+
+def trlwe_encrypt_level(level, seed, key, polys, alpha=None):
+    """TRLWE encryptions at level 1 or 2 (a then b, 2N words of the level's
+    torus type) of the polynomials in the rows of polys."""
+    polys = np.ascontiguousarray(np.atleast_2d(polys), _torus(level))
+    a = ALPHA[level] if alpha is None else alpha
+    out = np.zeros((polys.shape[0], 2 * polys.shape[1]), _torus(level))
+    for i in range(polys.shape[0]):
+        check(lib().tfhe_trlwe_encrypt_level(level, seed, i, _ptr(key), _ptr(polys[i]), a,
+                                             _ptr(out[i])), "tfhe_trlwe_encrypt_level")
+    return out
+
+
+def trlwe_phase(level, key, ct):
+    """Phase polynomials b - a*s (torus words) of the TRLWEs in the rows of ct."""
+    ct = np.ascontiguousarray(np.atleast_2d(ct), _torus(level))
+    out = np.zeros((ct.shape[0], ct.shape[1] // 2), _torus(level))
+    check(lib().tfhe_trlwe_phase(level, _ptr(key), _ptr(ct), ct.shape[0], _ptr(out)),
+          "tfhe_trlwe_phase")
+    return out
+
+
+def annihilate_key_gen(level, seed, key, alpha=None):
+    """Annihilation key of level 1 or 2 in the polynomial domain:
+    (nbit, l, 2, N) torus words (half TRGSWs of tau_d(s), d = 2^(bit+1) + 1)."""
+    words = lib().tfhe_annihilate_key_words(level)
+    if words == 0:
+        raise ValueError("annihilation level must be 1 or 2")
+    out = np.zeros(words, _torus(level))
+    a = ALPHA[level] if alpha is None else alpha
+    check(lib().tfhe_annihilate_key_gen(level, seed, _ptr(key), a, _ptr(out), words),
+          "tfhe_annihilate_key_gen")
+    return out
+
+
+def annihilate_key_load(level, ahk):
+    ahk = np.ascontiguousarray(ahk, _torus(level))
+    check(lib().cufhe_annihilate_key_load(level, _ptr(ahk), ahk.size), "cufhe_annihilate_key_load")
 
 
 def phase(level, key, ct):

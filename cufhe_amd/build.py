@@ -14,8 +14,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcufhe_amd.so")
 
-HIP_SOURCES = ["kernels_fft.hip", "kernels_ntt.hip", "kernels_ntt64.hip", "ks_gemm.hip",
-               "runtime.hip", "server.hip"]
+HIP_SOURCES = ["kernels_fft.hip", "kernels_ntt.hip", "kernels_ntt64.hip", "kernels_annihilate.hip",
+               "ks_gemm.hip", "runtime.hip", "server.hip"]
 CXX_SOURCES = ["tfhe_host.cpp"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

@@ -1,0 +1,140 @@
+// Annihilation key switching on the FFT backend (reference
+// src/annihilate_gpu.cu:676-803 AnnihilateKeySwitching*, with the kernels it
+// launches: :180-202 halving and sum, :436-462 / :557-603 EvalAuto, :245-360
+// the half external product).
+//
+// Per TRLWE c = (a, b) of ring size N = 2^nbit, nbit rounds, round `bit`
+// with d = 2^(bit+1) + 1:
+//   c >>= 1                                  (every torus word, logical shift)
+//   c += (0, tau_d(b)) - ExtProd_half(tau_d(a), AHK[bit])
+// tau_d is the automorphism p(X) -> p(X^d); the half external product
+// decomposes tau_d(a) into l gadget digits and multiplies digit q by the two
+// polynomials of the key's row q. After the nbit rounds the phase keeps the
+// constant coefficient of the input's phase and sums the others to ~0.
+//
+// MI355X layout: one workgroup of two waves per TRLWE runs all nbit rounds
+// with the TRLWE resident in LDS (the reference launches three kernels per
+// round and round-trips the TRLWE through global memory between them).
+// Wave w owns output component w. The l digit transforms of a round are split
+// over the two waves (digits 2p and 2p+1 transformed side by side, the
+// spectra exchanged through LDS), every wave multiplies all l spectra with
+// its component's key row and inverse-transforms only its own component.
+// tau_d is applied while reading: coefficient j of tau_d(p) is p[u] for
+// u = j d^-1 mod 2N below N and -p[u - N] otherwise, so no permuted copy is
+// stored. The key spectra (nbit x l x 2 polynomials, 0.33 MB at N = 1024,
+// 1.44 MB at N = 2048) stay in L2 across the batch.
+#include "br_fft.hpp"
+#include "launch.hpp"
+
+namespace cufhe_amd {
+
+template <class P>
+__global__ __launch_bounds__(128) void k_annihilate_fft(typename P::T* out, size_t out_stride,
+                                                        const typename P::T* in, size_t in_stride,
+                                                        const cplx* ahk, FFTTables tb)
+{
+    using T = typename P::T;
+    constexpr int N = P::n, H = N / 2, R = H / 64, L = P::l;
+    static_assert(L % 2 == 0, "digits are transformed in pairs, one per wave");
+    __shared__ T sc[2 * N];       // the TRLWE: a at [0, N), b at [N, 2N)
+    __shared__ cplx spec[2][H];   // per wave: transform scratch, then a digit spectrum
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const size_t job = blockIdx.x;
+
+    const T* src = in + job * in_stride;
+    for (int i = threadIdx.x; i < 2 * N; i += 128) sc[i] = src[i];
+    __syncthreads();
+
+    constexpr double scale = kTorusScale<T> / H;
+    T* const own = sc + wave * N;
+#pragma unroll 1
+    for (int bit = 0; bit < static_cast<int>(P::nbit); bit++) {
+        const uint32_t d = (2u << bit) + 1u;
+        // d^-1 mod 2^32 by Newton's iteration (d d = 1 mod 8; each step
+        // doubles the correct low bits), then mod 2N
+        uint32_t di = d;
+#pragma unroll
+        for (int k = 0; k < 4; k++) di *= 2u - d * di;
+        di &= 2u * N - 1u;
+        // coefficient j of tau_d(p >> 1)
+        auto tau = [di](const T* p, int j) -> T {
+            const uint32_t u = (static_cast<uint32_t>(j) * di) & (2u * N - 1u);
+            const T v = p[u & (N - 1)] >> 1;
+            return u < static_cast<uint32_t>(N) ? v : static_cast<T>(0 - v);
+        };
+
+        cplx acc[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) acc[r] = cplx{0.0, 0.0};
+#pragma unroll 1
+        for (int pr = 0; pr < L / 2; pr++) {
+            const int q = 2 * pr + wave;
+            cplx z[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int m = lane + 64 * r;
+                z[r] = cplx{static_cast<double>(Gadget<P>::digit(tau(sc, m), q)),
+                            static_cast<double>(Gadget<P>::digit(tau(sc, m + H), q))};
+            }
+            fft_forward<H>(z, spec[wave], tb);
+#pragma unroll
+            for (int r = 0; r < R; r++) lds_store(&spec[wave][r * 64 + lane], z[r]);
+            __syncthreads();  // both digit spectra of the pair written
+#pragma unroll
+            for (int w2 = 0; w2 < 2; w2++) {
+                const cplx* key =
+                    ahk + ((static_cast<size_t>(bit) * L + 2 * pr + w2) * 2 + wave) * H + lane;
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    cmac(acc[r], lds_load(&spec[w2][r * 64 + lane]), key[r * 64]);
+            }
+            __syncthreads();  // both waves done reading before the next pair's transforms
+        }
+        // Every read of the a half (both waves, in the digit loop) precedes the
+        // barrier above, and only wave 1 reads or writes the b half from here on.
+        fft_inverse<H>(acc, spec[wave], tb);
+        T nv[2 * R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int m = lane + 64 * r;
+            const T lo = double_to_torus<T>(acc[r].x * scale);
+            const T hi = double_to_torus<T>(acc[r].y * scale);
+            T vlo = static_cast<T>((own[m] >> 1) - lo);
+            T vhi = static_cast<T>((own[m + H] >> 1) - hi);
+            if (wave == 1) {
+                vlo += tau(own, m);
+                vhi += tau(own, m + H);
+            }
+            nv[r] = vlo;
+            nv[R + r] = vhi;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            own[lane + 64 * r] = nv[r];
+            own[lane + 64 * r + H] = nv[R + r];
+        }
+        __syncthreads();  // the round's TRLWE visible to both waves
+    }
+
+    T* dst = out + job * out_stride;
+    for (int i = threadIdx.x; i < 2 * N; i += 128) dst[i] = sc[i];
+}
+
+template <class P>
+void launch_annihilate_fft(typename P::T* out, size_t out_stride, const typename P::T* in,
+                           size_t in_stride, int batch, const cplx* ahk, const FourierKeys<P>& K,
+                           hipStream_t st)
+{
+    if (batch <= 0) return;
+    hipLaunchKernelGGL((k_annihilate_fft<P>), dim3(batch), dim3(128), 0, st, out, out_stride, in,
+                       in_stride, ahk, K.fft);
+}
+template void launch_annihilate_fft<lvl1param>(uint32_t*, size_t, const uint32_t*, size_t, int,
+                                               const cplx*, const FourierKeys<lvl1param>&,
+                                               hipStream_t);
+template void launch_annihilate_fft<lvl2param>(uint64_t*, size_t, const uint64_t*, size_t, int,
+                                               const cplx*, const FourierKeys<lvl2param>&,
+                                               hipStream_t);
+
+}  // namespace cufhe_amd

@@ -53,6 +53,13 @@ template <class P>
 void launch_cmux_fft(typename P::T* trlwe, const cplx* trgsw, uint32_t abar,
                      const FourierKeys<P>& K, hipStream_t st);
 
+// Annihilation key switching of `batch` TRLWEs (kernels_annihilate.hip);
+// strides in torus words, ahk: the key's nbit x l x 2 spectra.
+template <class P>
+void launch_annihilate_fft(typename P::T* out, size_t out_stride, const typename P::T* in,
+                           size_t in_stride, int batch, const cplx* ahk, const FourierKeys<P>& K,
+                           hipStream_t st);
+
 // NTT backend (lvl1): mode 0 plain CMUX, 1 key bundle, 2 block-binary.
 void launch_cmux_batch_fft(const CmuxBatch& batch, int n, const FourierKeys<lvl1param>& K,
                            hipStream_t st);

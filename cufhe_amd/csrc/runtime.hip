@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <mutex>
 #include <string>
@@ -92,6 +93,8 @@ struct State {
     Ntt64TableSet t64;
     LevelKeys<lvl1param> k1;
     LevelKeys<lvl2param> k2;
+    cplx* ahk1 = nullptr;  // annihilation key spectra, level 1 and 2
+    cplx* ahk2 = nullptr;
 };
 
 State& state()
@@ -400,6 +403,34 @@ int load_bk(LevelKeys<typename brP::targetP>& lk, const TableSet<typename brP::t
     (void)hipFree(tmp);
     if (e != hipSuccess) return fail(CUFHE_ERR_HIP, std::string("bk conversion: ") + hipGetErrorString(e));
     lk.kb = kb;
+    return CUFHE_OK;
+}
+
+// Annihilation key: nbit x l x 2 polynomials -> spectra in dst (allocated on
+// the first load of the level).
+template <class P>
+int load_annihilate_key(cplx*& dst, const TableSet<P>& ts, const void* ahk, size_t words)
+{
+    constexpr size_t npolys = static_cast<size_t>(P::nbit) * P::l * 2;
+    const size_t want = npolys * P::n;
+    if (words != want)
+        return fail(CUFHE_ERR_SIZE, "annihilation key has " + std::to_string(words) +
+                                        " words, expected " + std::to_string(want));
+    const size_t bytes = want * sizeof(typename P::T);
+    void* tmp = nullptr;
+    CUFHE_HIP_TRY(hipMalloc(&tmp, bytes));
+    hipError_t e = hipMemcpy(tmp, ahk, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !dst)
+        e = hipMalloc(reinterpret_cast<void**>(&dst), npolys * (P::n / 2) * sizeof(cplx));
+    if (e != hipSuccess) {
+        (void)hipFree(tmp);
+        return fail(CUFHE_ERR_HIP, std::string("annihilation key upload: ") + hipGetErrorString(e));
+    }
+    launch_to_fourier<P>(dst, tmp, static_cast<int>(npolys), false, keys_view(ts, nullptr), nullptr);
+    e = hipDeviceSynchronize();
+    (void)hipFree(tmp);
+    if (e != hipSuccess)
+        return fail(CUFHE_ERR_HIP, std::string("annihilation key conversion: ") + hipGetErrorString(e));
     return CUFHE_OK;
 }
 
@@ -945,6 +976,9 @@ int cufhe_cleanup(void)
     (void)hipDeviceSynchronize();
     free_keys(s.k1);
     free_keys(s.k2);
+    (void)hipFree(s.ahk1);
+    (void)hipFree(s.ahk2);
+    s.ahk1 = s.ahk2 = nullptr;
     free_tables(s.t1);
     free_tables(s.t2);
     free_ntt_tables(s.tn);
@@ -1885,6 +1919,71 @@ int cufhe_trgsw_to_fourier(int N, const void* d_trgsw, void* d_spectra, int coun
         return cufhe_ffnt_forward(N, d_trgsw, 0, d_spectra,
                                   count * TFHEpp::BootstrappingTRGSWRows<lvl2param> * 2, stream);
     return fail(CUFHE_ERR_UNSUPPORTED, "N must be 1024 or 2048");
+}
+
+int cufhe_annihilate_key_load(int level, const void* ahk, size_t words)
+{
+    if (!ahk) return fail(CUFHE_ERR_ARG, "null annihilation key");
+    State& s = state();
+    std::lock_guard<std::mutex> g(s.mu);
+    const int rc = ensure_init();
+    if (rc != CUFHE_OK) return rc;
+    if (level == 1) return load_annihilate_key(s.ahk1, s.t1, ahk, words);
+    if (level == 2) return load_annihilate_key(s.ahk2, s.t2, ahk, words);
+    return fail(CUFHE_ERR_ARG, "annihilation level must be 1 or 2");
+}
+
+int cufhe_annihilate_key_delete(int level)
+{
+    if (level != 1 && level != 2) return fail(CUFHE_ERR_ARG, "annihilation level must be 1 or 2");
+    State& s = state();
+    std::lock_guard<std::mutex> g(s.mu);
+    cplx*& k = level == 1 ? s.ahk1 : s.ahk2;
+    if (k) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(k);
+        k = nullptr;
+    }
+    return CUFHE_OK;
+}
+
+int cufhe_annihilate_key_switch_batch(int level, void* d_out, size_t out_stride, const void* d_in,
+                                      size_t in_stride, size_t batch, void* stream)
+{
+    if (level != 1 && level != 2) return fail(CUFHE_ERR_ARG, "annihilation level must be 1 or 2");
+    if (batch == 0) return CUFHE_OK;
+    if (!d_out || !d_in) return fail(CUFHE_ERR_ARG, "null pointer");
+    const size_t words = level == 1 ? 2 * lvl1param::n : 2 * lvl2param::n;
+    if (out_stride < words || in_stride < words)
+        return fail(CUFHE_ERR_ARG, "stride below the TRLWE size");
+    if (batch > static_cast<size_t>(std::numeric_limits<int>::max()))
+        return fail(CUFHE_ERR_ARG, "batch too large");
+    State& s = state();
+    const cplx* key;
+    {
+        std::lock_guard<std::mutex> g(s.mu);
+        const int rc = ensure_init();
+        if (rc != CUFHE_OK) return rc;
+        key = level == 1 ? s.ahk1 : s.ahk2;
+    }
+    if (!key) return fail(CUFHE_ERR_STATE, "no annihilation key loaded for this level");
+    if (level == 1)
+        launch_annihilate_fft<lvl1param>(static_cast<uint32_t*>(d_out), out_stride,
+                                         static_cast<const uint32_t*>(d_in), in_stride,
+                                         static_cast<int>(batch), key, keys_view(s.t1, nullptr),
+                                         as_stream(stream));
+    else
+        launch_annihilate_fft<lvl2param>(static_cast<uint64_t*>(d_out), out_stride,
+                                         static_cast<const uint64_t*>(d_in), in_stride,
+                                         static_cast<int>(batch), key, keys_view(s.t2, nullptr),
+                                         as_stream(stream));
+    return check_launch("annihilate");
+}
+
+int cufhe_annihilate_key_switch(int level, void* d_out, const void* d_in, void* stream)
+{
+    const size_t words = level == 2 ? 2 * lvl2param::n : 2 * lvl1param::n;
+    return cufhe_annihilate_key_switch_batch(level, d_out, words, d_in, words, 1, stream);
 }
 
 int cufhe_cmux_accumulate(int N, void* d_trlwe, const void* d_trgsw_spectra, uint32_t abar,

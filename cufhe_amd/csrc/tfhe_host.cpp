@@ -113,6 +113,7 @@ constexpr uint64_t kTagSecret = 1ULL << 56;
 constexpr uint64_t kTagBK = 2ULL << 56;
 constexpr uint64_t kTagKSK = 3ULL << 56;
 constexpr uint64_t kTagEnc = 4ULL << 56;
+constexpr uint64_t kTagAHK = 5ULL << 56;
 
 template <class T>
 T torus_uniform(ChaCha20& rng)
@@ -175,6 +176,53 @@ void trgsw_encrypt(typename P::T* out, int m, const typename P::T* key, ChaCha20
         const T h = static_cast<T>(1) << (std::numeric_limits<T>::digits - (digit + 1) * P::Bgbit);
         out[r * 2 * N + j * N] += static_cast<T>(m) * h;
     }
+}
+
+// p(X) -> p(X^d) in Z_{2^w}[X]/(X^N+1), d odd (TFHEpp Automorphism).
+template <class T>
+void automorphism(const T* in, uint32_t N, uint32_t d, T* out)
+{
+    for (uint32_t i = 0; i < N; i++) {
+        const uint32_t idx = i * d;
+        out[idx & (N - 1)] = (idx & N) ? static_cast<T>(0 - in[i]) : in[i];
+    }
+}
+
+// Half TRGSW of the polynomial p (TFHEpp halftrgswSymEncrypt): l TRLWE
+// encryptions of zero, row q's b component plus p * h_q (h_q = 2^(w-(q+1)Bgbit)).
+template <class P>
+void halftrgsw_encrypt(typename P::T* out, const typename P::T* p, const typename P::T* key,
+                       double alpha, ChaCha20& rng)
+{
+    using T = typename P::T;
+    constexpr uint32_t N = P::n;
+    for (uint32_t q = 0; q < P::l; q++) {
+        T* row = out + static_cast<size_t>(q) * 2 * N;
+        trlwe_encrypt_zero<P>(row, key, alpha, rng);
+        const T h = static_cast<T>(1) << (std::numeric_limits<T>::digits - (q + 1) * P::Bgbit);
+        for (uint32_t j = 0; j < N; j++) row[N + j] += p[j] * h;
+    }
+}
+
+// Annihilation key (reference src/annihilate_gpu.cu:605-628
+// AnnihilateKeyPolynomialGen): for bit < nbit and d = 2^(bit+1) + 1, the half
+// TRGSW of tau_d(s) under s. Layout: [bit][row q][component a, b][N].
+template <class P>
+void annihilate_key_gen(uint64_t seed, const typename P::T* key, double alpha, typename P::T* out)
+{
+    using T = typename P::T;
+    constexpr uint32_t N = P::n;
+    constexpr size_t per_bit = static_cast<size_t>(P::l) * 2 * N;
+    std::vector<std::thread> pool;
+    for (uint32_t bit = 0; bit < P::nbit; bit++) {
+        pool.emplace_back([=]() {
+            std::vector<T> autokey(N);
+            automorphism<T>(key, N, (2u << bit) + 1, autokey.data());
+            ChaCha20 rng(seed, kTagAHK | bit);
+            halftrgsw_encrypt<P>(out + bit * per_bit, autokey.data(), key, alpha, rng);
+        });
+    }
+    for (auto& th : pool) th.join();
 }
 
 template <class brP>
@@ -392,6 +440,75 @@ int tfhe_trlwe_encrypt(uint64_t seed, uint64_t index, const uint32_t* key, const
     ChaCha20 rng(seed, kTagEnc | index);
     trlwe_encrypt_zero<lvl1param>(ct, key, alpha, rng);
     for (uint32_t i = 0; i < lvl1param::n; i++) ct[lvl1param::n + i] += m[i];
+    return CUFHE_OK;
+}
+
+int tfhe_trlwe_encrypt_level(int level, uint64_t seed, uint64_t index, const void* key,
+                             const void* m, double alpha, void* ct)
+{
+    if (!key || !m || !ct) return CUFHE_ERR_ARG;
+    auto run = [&](auto tag) {
+        using P = decltype(tag);
+        using T = typename P::T;
+        ChaCha20 rng(seed, kTagEnc | index);
+        T* c = static_cast<T*>(ct);
+        trlwe_encrypt_zero<P>(c, static_cast<const T*>(key), alpha, rng);
+        for (uint32_t i = 0; i < P::n; i++) c[P::n + i] += static_cast<const T*>(m)[i];
+    };
+    if (level == 1)
+        run(lvl1param{});
+    else if (level == 2)
+        run(lvl2param{});
+    else
+        return CUFHE_ERR_ARG;
+    return CUFHE_OK;
+}
+
+int tfhe_trlwe_phase(int level, const void* key, const void* ct, size_t count, void* out)
+{
+    if (!key || (count && (!ct || !out))) return CUFHE_ERR_ARG;
+    auto run = [&](auto tag) {
+        using P = decltype(tag);
+        using T = typename P::T;
+        constexpr uint32_t N = P::n;
+        std::vector<T> as(N);
+        for (size_t c = 0; c < count; c++) {
+            const T* x = static_cast<const T*>(ct) + c * 2 * N;
+            std::fill(as.begin(), as.end(), T(0));
+            poly_mul_binary_acc<T>(as.data(), x, static_cast<const T*>(key), N);
+            T* o = static_cast<T*>(out) + c * N;
+            for (uint32_t i = 0; i < N; i++) o[i] = static_cast<T>(x[N + i] - as[i]);
+        }
+    };
+    if (level == 1)
+        run(lvl1param{});
+    else if (level == 2)
+        run(lvl2param{});
+    else
+        return CUFHE_ERR_ARG;
+    return CUFHE_OK;
+}
+
+size_t tfhe_annihilate_key_words(int level)
+{
+    if (level == 1) return static_cast<size_t>(lvl1param::nbit) * lvl1param::l * 2 * lvl1param::n;
+    if (level == 2) return static_cast<size_t>(lvl2param::nbit) * lvl2param::l * 2 * lvl2param::n;
+    return 0;
+}
+
+int tfhe_annihilate_key_gen(int level, uint64_t seed, const void* key, double alpha, void* out,
+                            size_t capacity_words)
+{
+    if (!key || !out || !(alpha >= 0.0)) return CUFHE_ERR_ARG;
+    const size_t words = tfhe_annihilate_key_words(level);
+    if (words == 0) return CUFHE_ERR_ARG;
+    if (capacity_words < words) return CUFHE_ERR_SIZE;
+    if (level == 1)
+        annihilate_key_gen<lvl1param>(seed, static_cast<const uint32_t*>(key), alpha,
+                                      static_cast<uint32_t*>(out));
+    else
+        annihilate_key_gen<lvl2param>(seed, static_cast<const uint64_t*>(key), alpha,
+                                      static_cast<uint64_t*>(out));
     return CUFHE_OK;
 }
 

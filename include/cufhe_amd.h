@@ -376,6 +376,30 @@ int cufhe_trgsw_to_fourier(int N, const void* d_trgsw, void* d_spectra, int coun
 int cufhe_cmux_accumulate(int N, void* d_trlwe, const void* d_trgsw_spectra, uint32_t abar,
                           void* stream);
 
+/* ---- Annihilation key switching (FFT backend; level 1: N = 1024, uint32
+ * words; level 2: N = 2048, uint64 words) -----------------------------------
+ * Key upload (reference include/annihilate_gpu.cuh:33-38,
+ * src/annihilate_gpu.cu:631-677 AnnihilateKeyPolynomialToDevice): the key in
+ * the polynomial domain as tfhe_annihilate_key_gen writes it (nbit x l x 2
+ * polynomials, tfhe_annihilate_key_words(level) words, host memory) is
+ * converted to spectra on the device; a new load replaces the old key. */
+int cufhe_annihilate_key_load(int level, const void* ahk, size_t words);
+/* DeleteAnnihilateKey (reference src/annihilate_gpu.cu:679-688). */
+int cufhe_annihilate_key_delete(int level);
+/* AnnihilateKeySwitching (reference src/annihilate_gpu.cu:787-797, :690-725):
+ * d_out = the annihilation key switch of the device TRLWE d_in (a then b,
+ * 2N words): nbit rounds of c >>= 1, c += EvalAuto_d(c) for
+ * d = 2^(bit+1) + 1. The phase keeps the input phase's constant coefficient;
+ * the other coefficients sum to ~0. d_out may equal d_in. Asynchronous on
+ * `stream`; needs no workspace. */
+int cufhe_annihilate_key_switch(int level, void* d_out, const void* d_in, void* stream);
+/* AnnihilateKeySwitchingBatchWithWorkspace (reference
+ * src/annihilate_gpu.cu:727-785): `batch` TRLWEs at the given strides (torus
+ * words, at least 2N) in one launch. Each output may equal its own input
+ * (the same pointers and strides); other overlaps are not allowed. */
+int cufhe_annihilate_key_switch_batch(int level, void* d_out, size_t out_stride, const void* d_in,
+                                      size_t in_stride, size_t batch, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,6 +71,25 @@ int tfhe_trlwe_encrypt(uint64_t seed, uint64_t index, const uint32_t* key_lvl1, 
 int tfhe_trgsw_encrypt(uint64_t seed, uint64_t index, const uint32_t* key_lvl1, int m,
                        uint32_t* ct_out);
 
+/* One TRLWE encryption (a then b, 2 x N words of the level's torus type)
+ * of the polynomial m under the level's key (level 1: uint32, N = 1024;
+ * level 2: uint64, N = 2048), noise alpha, stream (seed, index). */
+int tfhe_trlwe_encrypt_level(int level, uint64_t seed, uint64_t index, const void* key,
+                             const void* m, double alpha, void* ct_out);
+
+/* Phase b - a*s of count TRLWEs (N torus words each) at level 1 or 2. */
+int tfhe_trlwe_phase(int level, const void* key, const void* ct, size_t count, void* phase_out);
+
+/* Annihilation key of level 1 or 2 (reference src/annihilate_gpu.cu:605-628
+ * AnnihilateKeyPolynomialGen): for bit < nbit and d = 2^(bit+1) + 1 the half
+ * TRGSW (l TRLWE rows, TFHEpp halftrgswSymEncrypt) of the automorphism
+ * tau_d(s) of the level's key s, noise alpha (the level's alpha for TFHEpp's
+ * keys). Layout [bit][row][a, b][N] words of the level's torus type:
+ * tfhe_annihilate_key_words(level) words. */
+size_t tfhe_annihilate_key_words(int level);
+int tfhe_annihilate_key_gen(int level, uint64_t seed, const void* key, double alpha,
+                            void* ahk_out, size_t capacity_words);
+
 /* Phase of count TLWEs as a torus fraction in [-1/2, 1/2). */
 int tfhe_tlwe_phase(int level, const void* key, const void* ct, size_t count,
                     double* phase_out);

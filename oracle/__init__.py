@@ -62,6 +62,8 @@ def lib():
                                     ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "oracle_gate_batch_lvl1_ex": ([ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp, vp, vp,
                                        ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "oracle_automorphism": ([ctypes.c_int, vp, u32, vp], ctypes.c_int),
+        "oracle_annihilate": ([ctypes.c_int, ctypes.c_int, vp, vp, vp], ctypes.c_int),
         "oracle_set_mutation": ([ctypes.c_int], ctypes.c_int),
         "oracle_cpu_baseline_prepare": ([vp, ctypes.c_int], vp),
         "oracle_cpu_baseline_release": ([vp], None),
@@ -148,6 +150,24 @@ def external_product(level, mode, d, trgsw):
     return out
 
 
+def automorphism(level, poly, d):
+    """p(X) -> p(X^d) mod X^N + 1 (d odd) on one polynomial of the level."""
+    poly = np.ascontiguousarray(poly, np.uint32 if level == 1 else np.uint64)
+    out = np.zeros_like(poly)
+    _ok(lib().oracle_automorphism(level, _p(poly), d, _p(out)), "automorphism")
+    return out
+
+
+def annihilate(level, mode, trlwe, ahk):
+    """Annihilation key switching of one TRLWE (a then b) with the key's
+    nbit x l x 2 polynomials."""
+    trlwe = np.ascontiguousarray(trlwe, np.uint32 if level == 1 else np.uint64)
+    out = np.zeros_like(trlwe)
+    _ok(lib().oracle_annihilate(level, mode, _p(trlwe), _p(np.ascontiguousarray(ahk)), _p(out)),
+        "annihilate")
+    return out
+
+
 def blind_rotate(level, mode, kb, tl0, bk, mu):
     n = 1024 if level == 1 else 2048
     out = np.zeros(2 * n, np.uint32 if level == 1 else np.uint64)
@@ -228,6 +248,7 @@ class CpuBaseline:
 MUTATIONS = {
     "ntt_torus_offset": 1, "bar_off_by_one": 2, "b_rounded": 3, "bb_sign": 4,
     "bb_not_halved": 5, "ks_no_round": 6, "kb_order": 7, "a_truncated": 8,
+    "auto_sign": 9, "ah_no_halve": 10, "ah_key_row": 11,
 }
 
 

@@ -59,6 +59,9 @@ enum Mutation {
     MUT_KS_NO_ROUND = 6,       // key-switch rounding offset dropped
     MUT_KB_ORDER = 7,          // key-bundle members 3i+1 and 3i+2 swapped
     MUT_A_TRUNCATED = 8,       // mask coefficients mod-switched without rounding
+    MUT_AUTO_SIGN = 9,         // automorphism without the negacyclic sign
+    MUT_AH_NO_HALVE = 10,      // annihilation rounds without the halving
+    MUT_AH_KEY_ROW = 11,       // annihilation digits paired with the wrong key row
 };
 int g_mut = MUT_NONE;
 
@@ -830,6 +833,63 @@ void identity_key_switch(u32 dn, bool subset, const DT* in0, const DT* in1, i32 
     }
 }
 
+// ---------------------------------------------------------------- annihilation
+// p(X) -> p(X^d) in Z_{2^w}[X]/(X^N + 1) for odd d: coefficient i moves to
+// i*d mod 2N, negated when that lands in [N, 2N) (reference
+// src/annihilate_gpu.cu:204-219 __AutomorphismPolynomial__).
+template <class T>
+void automorphism(const T* in, u32 N, u32 d, T* out)
+{
+    for (u32 i = 0; i < N; i++) {
+        const u32 idx = i * d;
+        const bool neg = (idx & N) != 0 && g_mut != MUT_AUTO_SIGN;
+        out[idx & (N - 1)] = neg ? static_cast<T>(0 - in[i]) : in[i];
+    }
+}
+
+// Annihilation key switching of one TRLWE (reference src/annihilate_gpu.cu
+// :676-703 and the kernels it launches: :180-202 halving, :436-462 EvalAuto,
+// :196-202 the sum). nbit rounds, round `bit` with d = 2^(bit+1) + 1:
+//   c >>= 1 (every word, logical shift of the unsigned torus word);
+//   c += EvalAuto_d(c) = (0, tau_d(b)) - ExtProd_half(tau_d(a), AHK[bit]),
+// where the half external product decomposes tau_d(a) alone over the l rows
+// (each 2 polynomials) of AHK[bit] (:245-434). That equals the full external
+// product of the TRLWE (tau_d(a), 0) with a TRGSW whose rows l .. 2l-1 are
+// zero (every gadget digit of the zero polynomial is 0), which is how the
+// three arithmetic modes compute it here. ahk: nbit x l x 2 polynomials.
+template <class P>
+void annihilate(int mode, const typename P::T* in, const typename P::T* ahk, typename P::T* out)
+{
+    using T = typename P::T;
+    constexpr u32 N = P::N, L = P::l;
+    std::vector<T> c(in, in + 2 * N), d(2 * N, 0), tb(N), prod(2 * N), C(4 * L * N, 0);
+    for (u32 bit = 0; bit < P::nbit; bit++) {
+        if (g_mut != MUT_AH_NO_HALVE)
+            for (auto& v : c) v >>= 1;
+        const u32 dd = (2u << bit) + 1;
+        automorphism<T>(c.data(), N, dd, d.data());
+        automorphism<T>(c.data() + N, N, dd, tb.data());
+        const T* kb = ahk + static_cast<size_t>(bit) * 2 * L * N;
+        for (u32 q = 0; q < L; q++) {
+            const u32 src = g_mut == MUT_AH_KEY_ROW ? (q + 1) % L : q;
+            std::copy(kb + src * 2 * N, kb + (src + 1) * 2 * N, C.begin() + q * 2 * N);
+        }
+        if (mode == ORACLE_FFT)
+            extprod_fft<P>(d.data(), key_spectra<P>(C.data()), prod.data());
+        else if (mode == ORACLE_NTT) {
+            std::fill(prod.begin(), prod.end(), T(0));
+            extprod_ntt<P>(d.data(), key_mod<P>(C.data()), ntt_prime<P>(), prod.data(), true);
+        }
+        else
+            extprod_exact<P>(d.data(), C.data(), prod.data());
+        for (u32 i = 0; i < N; i++) {
+            c[i] += static_cast<T>(0 - prod[i]);
+            c[N + i] += static_cast<T>(tb[i] - prod[N + i]);
+        }
+    }
+    std::copy(c.begin(), c.end(), out);
+}
+
 struct GateLin {
     i32 ca, cb, off;
 };
@@ -1042,6 +1102,35 @@ int oracle_identity_key_switch(int level, int subset, const void* in0, const voi
     return 0;
 }
 
+// Automorphism X -> X^d (d odd) of one polynomial of the level's ring.
+int oracle_automorphism(int level, const void* in, u32 d, void* out)
+{
+    if ((d & 1u) == 0) return -1;
+    if (level == 1)
+        automorphism<u32>(static_cast<const u32*>(in), L1::N, d, static_cast<u32*>(out));
+    else if (level == 2)
+        automorphism<u64>(static_cast<const u64*>(in), L2::N, d, static_cast<u64*>(out));
+    else
+        return -1;
+    return 0;
+}
+
+// Annihilation key switching of one TRLWE (a then b) with the key in the
+// polynomial domain (nbit x l x 2 polynomials).
+int oracle_annihilate(int level, int mode, const void* in, const void* ahk, void* out)
+{
+    if (mode < ORACLE_EXACT || mode > ORACLE_NTT) return -1;
+    if (level == 1)
+        annihilate<L1>(mode, static_cast<const u32*>(in), static_cast<const u32*>(ahk),
+                       static_cast<u32*>(out));
+    else if (level == 2)
+        annihilate<L2>(mode, static_cast<const u64*>(in), static_cast<const u64*>(ahk),
+                       static_cast<u64*>(out));
+    else
+        return -1;
+    return 0;
+}
+
 // lvl1 gate API (IKS-BR order): lvl1 TLWEs in and out. kb: blind-rotation
 // variant (0 plain, 1 key bundle, 2 block-binary) plus 4 for a subset KSK.
 }  // extern "C"
@@ -1172,7 +1261,7 @@ int oracle_gate_batch_lvl1(int mode, int kb, const int32_t* ops, const u32* in0,
 // oracle). Used only by the tests that show the pinning checks catch them.
 int oracle_set_mutation(int m)
 {
-    if (m < 0 || m > MUT_A_TRUNCATED) return -1;
+    if (m < 0 || m > MUT_AH_KEY_ROW) return -1;
     g_mut = m;
     return 0;
 }
