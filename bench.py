@@ -582,6 +582,124 @@ def workload_nttbb(args):
     return res
 
 
+# ------------------------------------------------------------------ annihilation
+# Algorithmic FP64 flops of one annihilation key switch: nbit rounds x (l
+# forward + 2 inverse H-point FFTs at 5 H log2 H, + 2 l H complex MACs at 8).
+FLOP_PER_ANNIHILATE = {1: 10 * (4 * 5 * 512 * 9 + 2 * 2 * 512 * 8),      # 1,085,440
+                       2: 11 * (6 * 5 * 1024 * 10 + 2 * 4 * 1024 * 8)}  # 4,100,096
+
+
+def _annihilate_level(C, args, level, B, key):
+    import numpy as np
+
+    N = 1024 if level == 1 else 2048
+    T = np.uint32 if level == 1 else np.uint64
+    w = 32 if level == 1 else 64
+    ahk = C.annihilate_key_gen(level, args.seed, key)
+    C.annihilate_key_load(level, ahk)
+    rng = np.random.default_rng(args.seed + level)
+    bits = rng.integers(0, 2, B)
+    m = rng.integers(0, 2**63, (B, N), dtype=np.uint64)
+    m = m.astype(T) if level == 2 else (m >> np.uint64(31)).astype(T)
+    mu = T(1) << T(w - 3)
+    m[:, 0] = np.where(bits == 1, mu, T(0) - mu)  # the bit at X^0, junk elsewhere
+    log("level %d: encrypting %d TRLWEs" % (level, B))
+    ct = C.trlwe_encrypt_level(level, 900 + args.seed, key, m)
+    din = C.DeviceBuffer.from_numpy(ct)
+    dout = C.DeviceBuffer(ct.nbytes)
+    st = C.Stream()
+
+    def launch():
+        C.check(C.lib().cufhe_annihilate_key_switch_batch(level, dout.ptr, 2 * N, din.ptr, 2 * N,
+                                                          B, st.handle), "annihilate")
+    for _ in range(args.warmup):
+        launch()
+    st.synchronize()
+    evs = [(C.Event(), C.Event()) for _ in range(args.steps)]
+    C.check(C.lib().cufhe_synchronize(), "synchronize")
+    t0 = time.perf_counter()
+    for e0, e1 in evs:
+        e0.record(st)
+        launch()
+        e1.record(st)
+    st.synchronize()
+    C.check(C.lib().cufhe_synchronize(), "synchronize")
+    total = time.perf_counter() - t0
+    kms = [e0.elapsed_ms(e1) for e0, e1 in evs]
+    for e0, e1 in evs:
+        e0.destroy()
+        e1.destroy()
+    st.destroy()
+    out = dout.download(T, (B, 2 * N))
+    ph0 = C.trlwe_phase(level, key, out)[:, 0]
+    ph0 = ph0.view(np.int32) if level == 1 else ph0.view(np.int64)
+    failures = int(np.sum((ph0 > 0).astype(np.int64) != bits))
+    dev = (np.abs(ph0.astype(np.float64)) - float(mu)) / 2.0 ** w
+    oracle_diff = None
+    if not args.no_cpu:
+        import oracle as O
+        worst = 0
+        for i in (0, B - 1):
+            ref = O.annihilate(level, O.EXACT, ct[i], ahk)
+            d = C.trlwe_phase(level, key, out[i])[0] - C.trlwe_phase(level, key, ref)[0]
+            d = d.view(np.int32) if level == 1 else d.view(np.int64)
+            worst = max(worst, int(np.abs(d.astype(np.float64)).max()))
+        oracle_diff = worst / 2.0 ** w
+    k_s = statistics.mean(kms) / 1e3
+    achieved = FLOP_PER_ANNIHILATE[level] * B / k_s / 1e12
+    log("level %d: %d TRLWEs, kernel %.3f ms, %.0f /s, %.2f TFLOP/s FP64, %d decrypt failures"
+        % (level, B, k_s * 1e3, B / k_s, achieved, failures))
+    return {
+        "trlwes": B, "N": N, "annihilations_per_s": round(B * args.steps / total, 2),
+        "ms_per_step": round(total * 1e3 / args.steps, 3), "kernel_ms": round(k_s * 1e3, 3),
+        "decrypt_failures": failures, "max_const_coef_error": float(np.abs(dev).max()),
+        "max_phase_diff_vs_exact_oracle": oracle_diff,
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                     "traffic": None,
+                     "basis": "algorithmic FP64 flops %d per annihilation (FP64 VALU)"
+                              % FLOP_PER_ANNIHILATE[level]},
+    }, ct[:64], ahk
+
+
+def workload_annihilate(args):
+    import numpy as np
+
+    import cufhe_amd as C
+
+    B = args.batch
+    log("keys (seed 0)")
+    _, k1, k2 = C.secret_key_gen(0)
+    C.check(C.lib().cufhe_initialize(), "initialize")
+    l1, sample1, ahk1 = _annihilate_level(C, args, 1, B, k1)
+    l2, _, _ = _annihilate_level(C, args, 2, max(1, B // 4), k2)
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        log("cpu baseline: oracle FFT mode, one thread")
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < min(args.cpu_seconds, 10.0):
+            O.annihilate(1, O.FFT, sample1, ahk1)
+            n += len(sample1)
+        el = time.perf_counter() - t0
+        cpu = {"value": round(n / el, 2), "unit": "annihilations/s", "cores": 1, "kind": "port",
+               "sample": "%d level-1 annihilations (N=1024) in %.1f s, batches of %d: the "
+                         "oracle's FP64 FFT mode (oracle/oracle.cpp annihilate: key spectra once "
+                         "per batch, radix-2 FFT), one thread" % (n, el, len(sample1)),
+               "host_cpu_model": host_cpu()[0]}
+    res = base_result(args, l1["annihilations_per_s"], l1["ms_per_step"], "f64", {
+        "workload": "annihilation key switching of 4096 lvl1 TRLWEs (N=1024, l=2, Bgbit=8; "
+                    "10 rounds) in one launch, inputs resident in HBM; lvl2 (N=2048, l=4, "
+                    "Bgbit=10; 11 rounds) at a quarter of the batch",
+        "global_batch": B, "parallelism": "single GPU"})
+    res["metric"] = "lvl1 annihilation key switches/s @batch 4096 (TRLWE N=1024)"
+    res["unit"] = "annihilations/s"
+    res.update({"kernel_ms": l1["kernel_ms"], "decrypt_failures": l1["decrypt_failures"],
+                "lvl1": l1, "lvl2": l2, "roofline": l1["roofline"], "cpu_baseline": cpu})
+    del l1["roofline"]
+    return res
+
+
 # ------------------------------------------------------------------ serving
 def workload_serving(args):
     import threading
@@ -684,7 +802,7 @@ def workload_serving(args):
 
 
 WORKLOADS = {"kb4096": workload_kb4096, "lvl2mix": workload_lvl2mix, "nttbb": workload_nttbb,
-             "serving": workload_serving}
+             "serving": workload_serving, "annihilate": workload_annihilate}
 
 
 def main():

@@ -139,6 +139,10 @@ def lib():
         "tfhe_trlwe_phase": ([i32, vp, vp, sz, vp], i32),
         "tfhe_annihilate_key_words": ([i32], sz),
         "tfhe_annihilate_key_gen": ([i32, u64, vp, ctypes.c_double, vp, sz], i32),
+        "cufhe_event_create": ([ctypes.POINTER(vp)], i32),
+        "cufhe_event_destroy": ([vp], i32),
+        "cufhe_event_record": ([vp, vp], i32),
+        "cufhe_event_elapsed_ms": ([vp, vp, ctypes.POINTER(ctypes.c_float)], i32),
         "cufhe_annihilate_key_load": ([i32, vp, sz], i32),
         "cufhe_annihilate_key_delete": ([i32], i32),
         "cufhe_annihilate_key_switch": ([i32, vp, vp, vp], i32),
@@ -440,6 +444,30 @@ class Stream:
     def destroy(self):
         if self.handle is not None:
             lib().cufhe_stream_destroy(self.handle)
+            self.handle = None
+
+
+class Event:
+    """HIP event (timing kernels on a stream)."""
+
+    def __init__(self):
+        p = ctypes.c_void_p()
+        check(lib().cufhe_event_create(ctypes.byref(p)), "cufhe_event_create")
+        self.handle = p
+
+    def record(self, stream=None):
+        check(lib().cufhe_event_record(self.handle, _sp(stream)), "cufhe_event_record")
+
+    def elapsed_ms(self, end):
+        """Milliseconds from this event to `end` (waits for `end`)."""
+        ms = ctypes.c_float()
+        check(lib().cufhe_event_elapsed_ms(self.handle, end.handle, ctypes.byref(ms)),
+              "cufhe_event_elapsed_ms")
+        return ms.value
+
+    def destroy(self):
+        if self.handle is not None:
+            lib().cufhe_event_destroy(self.handle)
             self.handle = None
 
 

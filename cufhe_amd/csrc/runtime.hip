@@ -1018,6 +1018,37 @@ int cufhe_stream_destroy(void* stream)
     return CUFHE_OK;
 }
 
+int cufhe_event_create(void** event)
+{
+    if (!event) return fail(CUFHE_ERR_ARG, "null event handle");
+    hipEvent_t e;
+    CUFHE_HIP_TRY(hipEventCreate(&e));
+    *event = e;
+    return CUFHE_OK;
+}
+
+int cufhe_event_destroy(void* event)
+{
+    if (!event) return CUFHE_OK;
+    CUFHE_HIP_TRY(hipEventDestroy(static_cast<hipEvent_t>(event)));
+    return CUFHE_OK;
+}
+
+int cufhe_event_record(void* event, void* stream)
+{
+    if (!event) return fail(CUFHE_ERR_ARG, "null event");
+    CUFHE_HIP_TRY(hipEventRecord(static_cast<hipEvent_t>(event), as_stream(stream)));
+    return CUFHE_OK;
+}
+
+int cufhe_event_elapsed_ms(void* start, void* end, float* ms)
+{
+    if (!start || !end || !ms) return fail(CUFHE_ERR_ARG, "null pointer");
+    CUFHE_HIP_TRY(hipEventSynchronize(static_cast<hipEvent_t>(end)));
+    CUFHE_HIP_TRY(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(end)));
+    return CUFHE_OK;
+}
+
 int cufhe_stream_query(void* stream)
 {
     const hipError_t e = hipStreamQuery(as_stream(stream));

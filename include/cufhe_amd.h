@@ -125,6 +125,13 @@ int cufhe_stream_destroy(void* stream);
 /* 1 when all work on the stream is complete, 0 when work is pending. */
 int cufhe_stream_query(void* stream);
 int cufhe_stream_synchronize(void* stream);
+/* HIP events for timing kernels on a stream (measurement plumbing; the
+ * reference times with cudaEvent directly, e.g. test/test_perf.cc).
+ * cufhe_event_elapsed_ms waits for `end` and returns end - start in ms. */
+int cufhe_event_create(void** event);
+int cufhe_event_destroy(void* event);
+int cufhe_event_record(void* event, void* stream);
+int cufhe_event_elapsed_ms(void* start, void* end, float* ms);
 
 /* ---- memory (reference Ctxt ctor/dtor include/cufhe_gpu.cuh:78-121 and
  *      CtxtCopyH2D/D2H :203-217) --------------------------------------- */

@@ -243,11 +243,26 @@ inline void trlweSymEncrypt(TRLWE<lvl1param>& c, const Polynomial<lvl1param>& p,
         c[1][i] = flat[lvl1param::n + i];
     }
 }
+// TRLWE<P> is [a, b] of N words each, the flat layout of the C ABI.
 template <class P>
 void trlweSymEncrypt(TRLWE<P>& c, const Polynomial<P>& p, const Key<P>& key)
 {
-    static_assert(std::is_same_v<P, lvl1param>, "lvl1 TRLWE only");
-    trlweSymEncrypt(c, p, key);
+    static_assert(std::is_same_v<P, lvl1param> || std::is_same_v<P, lvl2param>,
+                  "lvl1 or lvl2 TRLWE");
+    detail::check(tfhe_trlwe_encrypt_level(detail::level<P>(), detail::encrypt_seed(),
+                                           detail::encrypt_counter()++, key.data(), p.data(), P::α,
+                                           c[0].data()),
+                  "tfhe_trlwe_encrypt_level");
+}
+
+// TFHEpp trlwePhase<P>: b - a * s.
+template <class P>
+Polynomial<P> trlwePhase(const TRLWE<P>& c, const Key<P>& key)
+{
+    Polynomial<P> ph;
+    detail::check(tfhe_trlwe_phase(detail::level<P>(), key.data(), c[0].data(), 1, ph.data()),
+                  "tfhe_trlwe_phase");
+    return ph;
 }
 
 // TFHEpp trgswSymEncrypt<lvl1param>(c, m, key): TRGSW of the integer m.

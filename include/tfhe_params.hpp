@@ -119,6 +119,9 @@ inline constexpr std::uint32_t BootstrappingTRGSWRows =
 
 template <class P>
 using TRGSW = std::array<TRLWE<P>, BootstrappingTRGSWRows<P>>;
+// l TRLWE rows: the gadget multiples of one polynomial added to b.
+template <class P>
+using HalfTRGSW = std::array<TRLWE<P>, P::l>;
 
 // Words of one TRGSW in the polynomial domain.
 template <class P>

@@ -63,7 +63,7 @@ def lib():
         "oracle_gate_batch_lvl1_ex": ([ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp, vp, vp,
                                        ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "oracle_automorphism": ([ctypes.c_int, vp, u32, vp], ctypes.c_int),
-        "oracle_annihilate": ([ctypes.c_int, ctypes.c_int, vp, vp, vp], ctypes.c_int),
+        "oracle_annihilate": ([ctypes.c_int, ctypes.c_int, vp, vp, vp, sz], ctypes.c_int),
         "oracle_set_mutation": ([ctypes.c_int], ctypes.c_int),
         "oracle_cpu_baseline_prepare": ([vp, ctypes.c_int], vp),
         "oracle_cpu_baseline_release": ([vp], None),
@@ -159,11 +159,12 @@ def automorphism(level, poly, d):
 
 
 def annihilate(level, mode, trlwe, ahk):
-    """Annihilation key switching of one TRLWE (a then b) with the key's
-    nbit x l x 2 polynomials."""
+    """Annihilation key switching of one TRLWE (a then b), or of each row of a
+    2-D array of them, with the key's nbit x l x 2 polynomials."""
     trlwe = np.ascontiguousarray(trlwe, np.uint32 if level == 1 else np.uint64)
     out = np.zeros_like(trlwe)
-    _ok(lib().oracle_annihilate(level, mode, _p(trlwe), _p(np.ascontiguousarray(ahk)), _p(out)),
+    n = trlwe.shape[0] if trlwe.ndim == 2 else 1
+    _ok(lib().oracle_annihilate(level, mode, _p(trlwe), _p(np.ascontiguousarray(ahk)), _p(out), n),
         "annihilate")
     return out
 

@@ -855,39 +855,78 @@ void automorphism(const T* in, u32 N, u32 d, T* out)
 // where the half external product decomposes tau_d(a) alone over the l rows
 // (each 2 polynomials) of AHK[bit] (:245-434). That equals the full external
 // product of the TRLWE (tau_d(a), 0) with a TRGSW whose rows l .. 2l-1 are
-// zero (every gadget digit of the zero polynomial is 0), which is how the
-// three arithmetic modes compute it here. ahk: nbit x l x 2 polynomials.
+// zero (every gadget digit of the zero polynomial is 0): the exact and NTT
+// modes compute it that way, the FFT mode skips the zero rows
+// (extprod_half_fft). ahk: nbit x l x 2 polynomials.
+// FP64 half external product (FFT mode): the rows r < l of extprod_fft in the
+// same order; the skipped rows l .. 2l-1 multiply zero digit spectra, whose
+// products would add exact zeros.
 template <class P>
-void annihilate(int mode, const typename P::T* in, const typename P::T* ahk, typename P::T* out)
+void extprod_half_fft(const typename P::T* t, const std::vector<std::vector<cd>>& Cspec,
+                      typename P::T* out)
+{
+    using T = typename P::T;
+    constexpr u32 N = P::N, H = N / 2;
+    const FFNT& F = ffnt(N);
+    std::vector<cd> acc[2] = {std::vector<cd>(H, 0.0), std::vector<cd>(H, 0.0)};
+    std::vector<double> tmp(N);
+    for (u32 q = 0; q < P::l; q++) {
+        for (u32 i = 0; i < N; i++) tmp[i] = static_cast<double>(gadget_digit<P>(t[i], q));
+        const std::vector<cd> z = F.forward(tmp.data());
+        for (u32 o = 0; o < 2; o++)
+            for (u32 b = 0; b < H; b++) acc[o][b] += z[b] * Cspec[q * 2 + o][b];
+    }
+    constexpr double scale = sizeof(T) == 4 ? 4294967296.0 : 18446744073709551616.0;
+    for (u32 o = 0; o < 2; o++) {
+        F.inverse(acc[o], tmp.data());
+        for (u32 i = 0; i < N; i++) out[o * N + i] = double_to_torus<T>(tmp[i] * scale);
+    }
+}
+
+// count TRLWEs, one after the other; the key is prepared once per call
+// (FFT mode: its spectra; NTT mode: its rows mapped to Z_P).
+template <class P>
+void annihilate(int mode, const typename P::T* in, const typename P::T* ahk, typename P::T* out,
+                size_t count)
 {
     using T = typename P::T;
     constexpr u32 N = P::N, L = P::l;
-    std::vector<T> c(in, in + 2 * N), d(2 * N, 0), tb(N), prod(2 * N), C(4 * L * N, 0);
+    std::vector<std::vector<T>> Cfull(P::nbit, std::vector<T>(4 * L * N, 0));
+    std::vector<std::vector<std::vector<cd>>> spec(P::nbit);
+    std::vector<std::vector<std::vector<u64>>> kmod(P::nbit);
     for (u32 bit = 0; bit < P::nbit; bit++) {
-        if (g_mut != MUT_AH_NO_HALVE)
-            for (auto& v : c) v >>= 1;
-        const u32 dd = (2u << bit) + 1;
-        automorphism<T>(c.data(), N, dd, d.data());
-        automorphism<T>(c.data() + N, N, dd, tb.data());
         const T* kb = ahk + static_cast<size_t>(bit) * 2 * L * N;
         for (u32 q = 0; q < L; q++) {
             const u32 src = g_mut == MUT_AH_KEY_ROW ? (q + 1) % L : q;
-            std::copy(kb + src * 2 * N, kb + (src + 1) * 2 * N, C.begin() + q * 2 * N);
+            std::copy(kb + src * 2 * N, kb + (src + 1) * 2 * N, Cfull[bit].begin() + q * 2 * N);
         }
-        if (mode == ORACLE_FFT)
-            extprod_fft<P>(d.data(), key_spectra<P>(C.data()), prod.data());
-        else if (mode == ORACLE_NTT) {
-            std::fill(prod.begin(), prod.end(), T(0));
-            extprod_ntt<P>(d.data(), key_mod<P>(C.data()), ntt_prime<P>(), prod.data(), true);
-        }
-        else
-            extprod_exact<P>(d.data(), C.data(), prod.data());
-        for (u32 i = 0; i < N; i++) {
-            c[i] += static_cast<T>(0 - prod[i]);
-            c[N + i] += static_cast<T>(tb[i] - prod[N + i]);
-        }
+        if (mode == ORACLE_FFT) spec[bit] = key_spectra<P>(Cfull[bit].data());
+        if (mode == ORACLE_NTT) kmod[bit] = key_mod<P>(Cfull[bit].data());
     }
-    std::copy(c.begin(), c.end(), out);
+    std::vector<T> c(2 * N), d(2 * N, 0), tb(N), prod(2 * N);
+    for (size_t j = 0; j < count; j++) {
+        std::copy(in + j * 2 * N, in + (j + 1) * 2 * N, c.begin());
+        for (u32 bit = 0; bit < P::nbit; bit++) {
+            if (g_mut != MUT_AH_NO_HALVE)
+                for (auto& v : c) v >>= 1;
+            const u32 dd = (2u << bit) + 1;
+            automorphism<T>(c.data(), N, dd, d.data());
+            automorphism<T>(c.data() + N, N, dd, tb.data());
+            if (mode == ORACLE_FFT)
+                extprod_half_fft<P>(d.data(), spec[bit], prod.data());
+            else if (mode == ORACLE_NTT) {
+                std::fill(prod.begin(), prod.end(), T(0));
+                extprod_ntt<P>(d.data(), kmod[bit], ntt_prime<P>(), prod.data(), true);
+            }
+            else
+                extprod_exact<P>(d.data(), Cfull[bit].data(), prod.data());
+            for (u32 i = 0; i < N; i++) {
+                c[i] += static_cast<T>(0 - prod[i]);
+                c[N + i] += static_cast<T>(tb[i] - prod[N + i]);
+            }
+        }
+        std::copy(c.begin(), c.end(), out + j * 2 * N);
+    }
 }
 
 struct GateLin {
@@ -1115,17 +1154,18 @@ int oracle_automorphism(int level, const void* in, u32 d, void* out)
     return 0;
 }
 
-// Annihilation key switching of one TRLWE (a then b) with the key in the
-// polynomial domain (nbit x l x 2 polynomials).
-int oracle_annihilate(int level, int mode, const void* in, const void* ahk, void* out)
+// Annihilation key switching of count TRLWEs (a then b each) with the key in
+// the polynomial domain (nbit x l x 2 polynomials).
+int oracle_annihilate(int level, int mode, const void* in, const void* ahk, void* out,
+                      size_t count)
 {
     if (mode < ORACLE_EXACT || mode > ORACLE_NTT) return -1;
     if (level == 1)
         annihilate<L1>(mode, static_cast<const u32*>(in), static_cast<const u32*>(ahk),
-                       static_cast<u32*>(out));
+                       static_cast<u32*>(out), count);
     else if (level == 2)
         annihilate<L2>(mode, static_cast<const u64*>(in), static_cast<const u64*>(ahk),
-                       static_cast<u64*>(out));
+                       static_cast<u64*>(out), count);
     else
         return -1;
     return 0;

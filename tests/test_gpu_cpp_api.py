@@ -48,3 +48,12 @@ def test_cpp_many_stream_stress():
     decrypts correctly."""
     p = _run("test_intensive_gpu.bin")
     assert p.returncode == 0 and "ALL PASS" in p.stdout
+
+
+def test_cpp_annihilate_api():
+    """AnnihilateKeyPolynomialGen / ToDevice / AnnihilateKeySwitching /
+    BatchWithWorkspace through include/annihilate_gpu.hpp at lvl1param and
+    lvl2param, the reference test's three input cases (reference
+    test/test_annihilate_gpu.cu)."""
+    p = _run("test_annihilate_gpu.bin")
+    assert p.returncode == 0 and "ALL ANNIHILATE TESTS PASSED" in p.stdout
