@@ -589,14 +589,14 @@ FLOP_PER_ANNIHILATE = {1: 10 * (4 * 5 * 512 * 9 + 2 * 2 * 512 * 8),      # 1,085
                        2: 11 * (6 * 5 * 1024 * 10 + 2 * 4 * 1024 * 8)}  # 4,100,096
 
 
-def _annihilate_level(C, args, level, B, key):
+def _annihilate_level(C, args, level, B, key, backend=0):
     import numpy as np
 
     N = 1024 if level == 1 else 2048
     T = np.uint32 if level == 1 else np.uint64
     w = 32 if level == 1 else 64
     ahk = C.annihilate_key_gen(level, args.seed, key)
-    C.annihilate_key_load(level, ahk)
+    C.annihilate_key_load(level, ahk, backend=backend)
     rng = np.random.default_rng(args.seed + level)
     bits = rng.integers(0, 2, B)
     m = rng.integers(0, 2**63, (B, N), dtype=np.uint64)
@@ -635,28 +635,32 @@ def _annihilate_level(C, args, level, B, key):
     ph0 = ph0.view(np.int32) if level == 1 else ph0.view(np.int64)
     failures = int(np.sum((ph0 > 0).astype(np.int64) != bits))
     dev = (np.abs(ph0.astype(np.float64)) - float(mu)) / 2.0 ** w
-    oracle_diff = None
+    oracle_cmp = None
     if not args.no_cpu:
         import oracle as O
-        worst = 0
+        worst, same = 0, 0
+        mode = O.NTT if backend else O.EXACT
         for i in (0, B - 1):
-            ref = O.annihilate(level, O.EXACT, ct[i], ahk)
+            ref = O.annihilate(level, mode, ct[i], ahk)
+            same += int(np.array_equal(out[i], ref))
             d = C.trlwe_phase(level, key, out[i])[0] - C.trlwe_phase(level, key, ref)[0]
             d = d.view(np.int32) if level == 1 else d.view(np.int64)
             worst = max(worst, int(np.abs(d.astype(np.float64)).max()))
-        oracle_diff = worst / 2.0 ** w
+        oracle_cmp = {"oracle_mode": "ntt" if backend else "exact", "sampled": 2,
+                      "identical_words": same, "max_phase_diff": worst / 2.0 ** w}
     k_s = statistics.mean(kms) / 1e3
     achieved = FLOP_PER_ANNIHILATE[level] * B / k_s / 1e12
-    log("level %d: %d TRLWEs, kernel %.3f ms, %.0f /s, %.2f TFLOP/s FP64, %d decrypt failures"
-        % (level, B, k_s * 1e3, B / k_s, achieved, failures))
+    log("level %d%s: %d TRLWEs, kernel %.3f ms, %.0f /s, %.2f TFLOP/s FP64, %d decrypt failures, "
+        "vs oracle %s" % (level, " (NTT)" if backend else "", B, k_s * 1e3, B / k_s, achieved,
+                          failures, oracle_cmp))
     return {
         "trlwes": B, "N": N, "annihilations_per_s": round(B * args.steps / total, 2),
         "ms_per_step": round(total * 1e3 / args.steps, 3), "kernel_ms": round(k_s * 1e3, 3),
         "decrypt_failures": failures, "max_const_coef_error": float(np.abs(dev).max()),
-        "max_phase_diff_vs_exact_oracle": oracle_diff,
+        "vs_oracle": oracle_cmp,
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": None,
+                     "traffic": measured_traffic("annihilate") if level == 1 and not backend else None,
                      "basis": "algorithmic FP64 flops %d per annihilation (FP64 VALU)"
                               % FLOP_PER_ANNIHILATE[level]},
     }, ct[:64], ahk
@@ -673,6 +677,9 @@ def workload_annihilate(args):
     C.check(C.lib().cufhe_initialize(), "initialize")
     l1, sample1, ahk1 = _annihilate_level(C, args, 1, B, k1)
     l2, _, _ = _annihilate_level(C, args, 2, max(1, B // 4), k2)
+    l2n, _, _ = _annihilate_level(C, args, 2, max(1, B // 4), k2, backend=C.BACKEND_NTT)
+    l2n["backend"] = "Goldilocks NTT (exact mod P)"
+    del l2n["roofline"]
     cpu = None
     if not args.no_cpu:
         import oracle as O
@@ -695,7 +702,8 @@ def workload_annihilate(args):
     res["metric"] = "lvl1 annihilation key switches/s @batch 4096 (TRLWE N=1024)"
     res["unit"] = "annihilations/s"
     res.update({"kernel_ms": l1["kernel_ms"], "decrypt_failures": l1["decrypt_failures"],
-                "lvl1": l1, "lvl2": l2, "roofline": l1["roofline"], "cpu_baseline": cpu})
+                "lvl1": l1, "lvl2": l2, "lvl2_ntt": l2n, "roofline": l1["roofline"],
+                "cpu_baseline": cpu})
     del l1["roofline"]
     return res
 

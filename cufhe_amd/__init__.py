@@ -144,6 +144,7 @@ def lib():
         "cufhe_event_record": ([vp, vp], i32),
         "cufhe_event_elapsed_ms": ([vp, vp, ctypes.POINTER(ctypes.c_float)], i32),
         "cufhe_annihilate_key_load": ([i32, vp, sz], i32),
+        "cufhe_annihilate_key_load_ex": ([i32, vp, sz, i32], i32),
         "cufhe_annihilate_key_delete": ([i32], i32),
         "cufhe_annihilate_key_switch": ([i32, vp, vp, vp], i32),
         "cufhe_annihilate_key_switch_batch": ([i32, vp, sz, vp, sz, sz, vp], i32),
@@ -363,9 +364,10 @@ def annihilate_key_gen(level, seed, key, alpha=None):
     return out
 
 
-def annihilate_key_load(level, ahk):
+def annihilate_key_load(level, ahk, backend=BACKEND_FFT):
     ahk = np.ascontiguousarray(ahk, _torus(level))
-    check(lib().cufhe_annihilate_key_load(level, _ptr(ahk), ahk.size), "cufhe_annihilate_key_load")
+    check(lib().cufhe_annihilate_key_load_ex(level, _ptr(ahk), ahk.size, backend),
+          "cufhe_annihilate_key_load_ex")
 
 
 def phase(level, key, ct):

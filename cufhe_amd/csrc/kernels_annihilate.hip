@@ -23,7 +23,13 @@
 // u = j d^-1 mod 2N below N and -p[u - N] otherwise, so no permuted copy is
 // stored. The key spectra (nbit x l x 2 polynomials, 0.33 MB at N = 1024,
 // 1.44 MB at N = 2048) stay in L2 across the batch.
+//
+// k_annihilate_ntt64 is the same schedule on the lvl2 NTT backend (the
+// reference's USE_FFT=OFF build, src/annihilate_gpu.cu:362-434): digits as
+// signed residues mod the Goldilocks prime, the key as torus_to_ntt_mod of
+// its words, products exact mod P, results back through ntt_mod_to_torus64.
 #include "br_fft.hpp"
+#include "br_ntt64.hpp"
 #include "launch.hpp"
 
 namespace cufhe_amd {
@@ -119,6 +125,91 @@ __global__ __launch_bounds__(128) void k_annihilate_fft(typename P::T* out, size
 
     T* dst = out + job * out_stride;
     for (int i = threadIdx.x; i < 2 * N; i += 128) dst[i] = sc[i];
+}
+
+__global__ __launch_bounds__(128) void k_annihilate_ntt64(uint64_t* out, size_t out_stride,
+                                                          const uint64_t* in, size_t in_stride,
+                                                          const uint64_t* ahk, Ntt64Tables tb)
+{
+    using P = lvl2param;
+    constexpr int N = P::n, L = P::l;
+    static_assert(L % 2 == 0, "digits are transformed in pairs, one per wave");
+    __shared__ uint64_t sc[2 * N];
+    __shared__ uint64_t buf[2][kNtt64Buf];  // per wave: transform scratch, then its digit spectrum
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const size_t job = blockIdx.x;
+
+    const uint64_t* src = in + job * in_stride;
+    for (int i = threadIdx.x; i < 2 * N; i += 128) sc[i] = src[i];
+    __syncthreads();
+
+    uint64_t* const own = sc + wave * N;
+#pragma unroll 1
+    for (int bit = 0; bit < static_cast<int>(P::nbit); bit++) {
+        const uint32_t d = (2u << bit) + 1u;
+        uint32_t di = d;
+#pragma unroll
+        for (int k = 0; k < 4; k++) di *= 2u - d * di;
+        di &= 2u * N - 1u;
+        auto tau = [di](const uint64_t* p, int j) -> uint64_t {
+            const uint32_t u = (static_cast<uint32_t>(j) * di) & (2u * N - 1u);
+            const uint64_t v = p[u & (N - 1)] >> 1;
+            return u < static_cast<uint32_t>(N) ? v : 0 - v;
+        };
+        uint64_t sp[32];
+#pragma unroll
+        for (int r = 0; r < 32; r++) sp[r] = 0;
+#pragma unroll 1
+        for (int pr = 0; pr < L / 2; pr++) {
+            const int q = 2 * pr + wave;
+            uint64_t x[32];
+#pragma unroll
+            for (int r = 0; r < 32; r++)
+                x[r] = signed_to_gold(Gadget<P>::digit(tau(sc, lane + 64 * r), q));
+            ntt64_forward(x, buf[wave], tb, lane);
+#pragma unroll
+            for (int r = 0; r < 32; r++) buf[wave][ntt64_addr(32 * lane + r)] = x[r];
+            __syncthreads();  // both digit spectra of the pair written
+#pragma unroll
+            for (int w2 = 0; w2 < 2; w2++) {
+                // key word 64 r + lane holds transform position 32 lane + r
+                const uint64_t* key =
+                    ahk + ((static_cast<size_t>(bit) * L + 2 * pr + w2) * 2 + wave) * N + lane;
+                const uint64_t* spec = buf[w2] + 33 * lane;
+#pragma unroll
+                for (int r = 0; r < 32; r++)
+                    sp[r] = gold::add(sp[r], gold::mul(spec[r], key[64 * r]));
+            }
+            __syncthreads();  // both waves done reading before the next pair's transforms
+        }
+        ntt64_inverse(sp, buf[wave], tb, lane);
+        // lane holds coefficients lane + 64 r; the a half was last read before
+        // the barrier above, the b half is read and written by wave 1 only
+        uint64_t nv[32];
+#pragma unroll
+        for (int r = 0; r < 32; r++) {
+            const int e = lane + 64 * r;
+            uint64_t v = (own[e] >> 1) - gold_to_torus64(sp[r]);
+            if (wave == 1) v += tau(own, e);
+            nv[r] = v;
+        }
+#pragma unroll
+        for (int r = 0; r < 32; r++) own[lane + 64 * r] = nv[r];
+        __syncthreads();
+    }
+
+    uint64_t* dst = out + job * out_stride;
+    for (int i = threadIdx.x; i < 2 * N; i += 128) dst[i] = sc[i];
+}
+
+void launch_annihilate_ntt64(uint64_t* out, size_t out_stride, const uint64_t* in,
+                             size_t in_stride, int batch, const uint64_t* ahk,
+                             const Ntt64Tables& tb, hipStream_t st)
+{
+    if (batch <= 0) return;
+    hipLaunchKernelGGL(k_annihilate_ntt64, dim3(batch), dim3(128), 0, st, out, out_stride, in,
+                       in_stride, ahk, tb);
 }
 
 template <class P>

@@ -59,6 +59,11 @@ template <class P>
 void launch_annihilate_fft(typename P::T* out, size_t out_stride, const typename P::T* in,
                            size_t in_stride, int batch, const cplx* ahk, const FourierKeys<P>& K,
                            hipStream_t st);
+// lvl2 on the Goldilocks NTT backend; ahk: the key's nbit x l x 2 transforms
+// (launch_to_ntt64 layout).
+void launch_annihilate_ntt64(uint64_t* out, size_t out_stride, const uint64_t* in,
+                             size_t in_stride, int batch, const uint64_t* ahk,
+                             const Ntt64Tables& tb, hipStream_t st);
 
 // NTT backend (lvl1): mode 0 plain CMUX, 1 key bundle, 2 block-binary.
 void launch_cmux_batch_fft(const CmuxBatch& batch, int n, const FourierKeys<lvl1param>& K,

@@ -95,6 +95,7 @@ struct State {
     LevelKeys<lvl2param> k2;
     cplx* ahk1 = nullptr;  // annihilation key spectra, level 1 and 2
     cplx* ahk2 = nullptr;
+    uint64_t* ahk2_ntt = nullptr;  // level 2 on the NTT backend (exclusive with ahk2)
 };
 
 State& state()
@@ -427,6 +428,33 @@ int load_annihilate_key(cplx*& dst, const TableSet<P>& ts, const void* ahk, size
         return fail(CUFHE_ERR_HIP, std::string("annihilation key upload: ") + hipGetErrorString(e));
     }
     launch_to_fourier<P>(dst, tmp, static_cast<int>(npolys), false, keys_view(ts, nullptr), nullptr);
+    e = hipDeviceSynchronize();
+    (void)hipFree(tmp);
+    if (e != hipSuccess)
+        return fail(CUFHE_ERR_HIP, std::string("annihilation key conversion: ") + hipGetErrorString(e));
+    return CUFHE_OK;
+}
+
+// Annihilation key on the lvl2 NTT backend: torus_to_ntt_mod and the forward
+// transform per polynomial (the bootstrapping key's launch_to_ntt64 layout).
+int load_annihilate_key_ntt64(uint64_t*& dst, const Ntt64TableSet& ts, const void* ahk, size_t words)
+{
+    using P = lvl2param;
+    constexpr size_t npolys = static_cast<size_t>(P::nbit) * P::l * 2;
+    const size_t want = npolys * P::n;
+    if (words != want)
+        return fail(CUFHE_ERR_SIZE, "annihilation key has " + std::to_string(words) +
+                                        " words, expected " + std::to_string(want));
+    const size_t bytes = want * sizeof(uint64_t);
+    void* tmp = nullptr;
+    CUFHE_HIP_TRY(hipMalloc(&tmp, bytes));
+    hipError_t e = hipMemcpy(tmp, ahk, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !dst) e = hipMalloc(reinterpret_cast<void**>(&dst), bytes);
+    if (e != hipSuccess) {
+        (void)hipFree(tmp);
+        return fail(CUFHE_ERR_HIP, std::string("annihilation key upload: ") + hipGetErrorString(e));
+    }
+    launch_to_ntt64(dst, static_cast<const uint64_t*>(tmp), static_cast<int>(npolys), ts.tb, nullptr);
     e = hipDeviceSynchronize();
     (void)hipFree(tmp);
     if (e != hipSuccess)
@@ -978,7 +1006,9 @@ int cufhe_cleanup(void)
     free_keys(s.k2);
     (void)hipFree(s.ahk1);
     (void)hipFree(s.ahk2);
+    (void)hipFree(s.ahk2_ntt);
     s.ahk1 = s.ahk2 = nullptr;
+    s.ahk2_ntt = nullptr;
     free_tables(s.t1);
     free_tables(s.t2);
     free_ntt_tables(s.tn);
@@ -1952,16 +1982,38 @@ int cufhe_trgsw_to_fourier(int N, const void* d_trgsw, void* d_spectra, int coun
     return fail(CUFHE_ERR_UNSUPPORTED, "N must be 1024 or 2048");
 }
 
-int cufhe_annihilate_key_load(int level, const void* ahk, size_t words)
+int cufhe_annihilate_key_load_ex(int level, const void* ahk, size_t words, int backend)
 {
     if (!ahk) return fail(CUFHE_ERR_ARG, "null annihilation key");
+    if (level != 1 && level != 2) return fail(CUFHE_ERR_ARG, "annihilation level must be 1 or 2");
+    if (backend != CUFHE_BACKEND_FFT && backend != CUFHE_BACKEND_NTT)
+        return fail(CUFHE_ERR_ARG, "backend must be CUFHE_BACKEND_FFT or CUFHE_BACKEND_NTT");
+    if (level == 1 && backend == CUFHE_BACKEND_NTT)
+        return fail(CUFHE_ERR_UNSUPPORTED, "level-1 annihilation runs on the FFT backend only");
     State& s = state();
     std::lock_guard<std::mutex> g(s.mu);
     const int rc = ensure_init();
     if (rc != CUFHE_OK) return rc;
     if (level == 1) return load_annihilate_key(s.ahk1, s.t1, ahk, words);
-    if (level == 2) return load_annihilate_key(s.ahk2, s.t2, ahk, words);
-    return fail(CUFHE_ERR_ARG, "annihilation level must be 1 or 2");
+    if (backend == CUFHE_BACKEND_FFT) {
+        const int r2 = load_annihilate_key(s.ahk2, s.t2, ahk, words);
+        if (r2 == CUFHE_OK && s.ahk2_ntt) {
+            (void)hipFree(s.ahk2_ntt);
+            s.ahk2_ntt = nullptr;
+        }
+        return r2;
+    }
+    const int r2 = load_annihilate_key_ntt64(s.ahk2_ntt, s.t64, ahk, words);
+    if (r2 == CUFHE_OK && s.ahk2) {
+        (void)hipFree(s.ahk2);
+        s.ahk2 = nullptr;
+    }
+    return r2;
+}
+
+int cufhe_annihilate_key_load(int level, const void* ahk, size_t words)
+{
+    return cufhe_annihilate_key_load_ex(level, ahk, words, CUFHE_BACKEND_FFT);
 }
 
 int cufhe_annihilate_key_delete(int level)
@@ -1969,11 +2021,16 @@ int cufhe_annihilate_key_delete(int level)
     if (level != 1 && level != 2) return fail(CUFHE_ERR_ARG, "annihilation level must be 1 or 2");
     State& s = state();
     std::lock_guard<std::mutex> g(s.mu);
-    cplx*& k = level == 1 ? s.ahk1 : s.ahk2;
-    if (k) {
-        (void)hipDeviceSynchronize();
-        (void)hipFree(k);
-        k = nullptr;
+    (void)hipDeviceSynchronize();
+    if (level == 1) {
+        (void)hipFree(s.ahk1);
+        s.ahk1 = nullptr;
+    }
+    else {
+        (void)hipFree(s.ahk2);
+        (void)hipFree(s.ahk2_ntt);
+        s.ahk2 = nullptr;
+        s.ahk2_ntt = nullptr;
     }
     return CUFHE_OK;
 }
@@ -1991,14 +2048,20 @@ int cufhe_annihilate_key_switch_batch(int level, void* d_out, size_t out_stride,
         return fail(CUFHE_ERR_ARG, "batch too large");
     State& s = state();
     const cplx* key;
+    const uint64_t* key_ntt;
     {
         std::lock_guard<std::mutex> g(s.mu);
         const int rc = ensure_init();
         if (rc != CUFHE_OK) return rc;
         key = level == 1 ? s.ahk1 : s.ahk2;
+        key_ntt = level == 2 ? s.ahk2_ntt : nullptr;
     }
-    if (!key) return fail(CUFHE_ERR_STATE, "no annihilation key loaded for this level");
-    if (level == 1)
+    if (!key && !key_ntt) return fail(CUFHE_ERR_STATE, "no annihilation key loaded for this level");
+    if (key_ntt)
+        launch_annihilate_ntt64(static_cast<uint64_t*>(d_out), out_stride,
+                                static_cast<const uint64_t*>(d_in), in_stride,
+                                static_cast<int>(batch), key_ntt, s.t64.tb, as_stream(stream));
+    else if (level == 1)
         launch_annihilate_fft<lvl1param>(static_cast<uint32_t*>(d_out), out_stride,
                                          static_cast<const uint32_t*>(d_in), in_stride,
                                          static_cast<int>(batch), key, keys_view(s.t1, nullptr),

@@ -5,7 +5,8 @@
 // TFHEpp::lvl1param and TFHEpp::lvl2param, the two rings the reference
 // instantiates besides its AH* parameter sets (src/annihilate_gpu.cu:805-815).
 //
-// Differences a caller can see: the FFT backend at both levels; the kernel
+// Differences a caller can see: the FFT backend at both levels, the NTT
+// backend at lvl2param as a run-time choice instead of a build option; the kernel
 // keeps the TRLWE in LDS, so the workspace arguments of the *WithWorkspace
 // forms are accepted and not used, and AnnihilateKeySwitching allocates
 // nothing; one GPU per process (gpuNum must be 0 or 1 as a count, 0 as an
@@ -72,16 +73,18 @@ void AnnihilateKeyPolynomialGen(AnnihilateKeyPolynomial<P>& ahk, const TFHEpp::S
 }
 
 // Transforms the key on the GPU and keeps it for the level (replacing a
-// previous one).
+// previous one). backend: CUFHE_BACKEND_FFT (the reference's default build)
+// or, for lvl2param, CUFHE_BACKEND_NTT (its USE_FFT=OFF build).
 template <class P>
-void AnnihilateKeyPolynomialToDevice(const AnnihilateKeyPolynomial<P>& ahk, const int gpuNum)
+void AnnihilateKeyPolynomialToDevice(const AnnihilateKeyPolynomial<P>& ahk, const int gpuNum,
+                                     const int backend = CUFHE_BACKEND_FFT)
 {
     if (gpuNum != 1) {
         std::fprintf(stderr, "cufhe: AnnihilateKeyPolynomialToDevice: one GPU per process\n");
         std::exit(-1);
     }
-    detail_ah::check(cufhe_annihilate_key_load(detail_ah::level<P>(), &ahk[0][0][0][0][0],
-                                               detail_ah::words<P>()),
+    detail_ah::check(cufhe_annihilate_key_load_ex(detail_ah::level<P>(), &ahk[0][0][0][0][0],
+                                                  detail_ah::words<P>(), backend),
                      "AnnihilateKeyPolynomialToDevice");
 }
 

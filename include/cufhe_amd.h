@@ -391,6 +391,12 @@ int cufhe_cmux_accumulate(int N, void* d_trlwe, const void* d_trgsw_spectra, uin
  * polynomials, tfhe_annihilate_key_words(level) words, host memory) is
  * converted to spectra on the device; a new load replaces the old key. */
 int cufhe_annihilate_key_load(int level, const void* ahk, size_t words);
+/* The same with the backend: CUFHE_BACKEND_FFT (both levels) or, at level 2,
+ * CUFHE_BACKEND_NTT (the Goldilocks prime: the reference's USE_FFT=OFF
+ * arithmetic, src/annihilate_gpu.cu:147-178 and :362-434, exact mod P with
+ * its torus <-> Z_P conversions). The last load of a level decides the
+ * backend of its key switches. */
+int cufhe_annihilate_key_load_ex(int level, const void* ahk, size_t words, int backend);
 /* DeleteAnnihilateKey (reference src/annihilate_gpu.cu:679-688). */
 int cufhe_annihilate_key_delete(int level);
 /* AnnihilateKeySwitching (reference src/annihilate_gpu.cu:787-797, :690-725):

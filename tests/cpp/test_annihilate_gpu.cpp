@@ -23,7 +23,7 @@ using namespace TFHEpp;
 namespace {
 
 template <class P>
-bool run_case(const char* name)
+bool run_case(const char* name, int backend = CUFHE_BACKEND_FFT)
 {
     using T = typename P::T;
     constexpr uint32_t N = P::n;
@@ -31,7 +31,7 @@ bool run_case(const char* name)
     auto sk = std::make_unique<SecretKey>(11);
     auto ahk = std::make_unique<cufhe::AnnihilateKeyPolynomial<P>>();
     cufhe::AnnihilateKeyPolynomialGen<P>(*ahk, *sk);
-    cufhe::AnnihilateKeyPolynomialToDevice<P>(*ahk, 1);
+    cufhe::AnnihilateKeyPolynomialToDevice<P>(*ahk, 1, backend);
 
     constexpr size_t bytes = sizeof(TRLWE<P>);
     T *d_in = nullptr, *d_out = nullptr, *d_ws = nullptr;
@@ -104,6 +104,7 @@ int main()
     cufhe::Initialize();
     bool ok = run_case<lvl1param>("lvl1param");
     ok = run_case<lvl2param>("lvl2param") && ok;
+    ok = run_case<lvl2param>("lvl2param, NTT backend", CUFHE_BACKEND_NTT) && ok;
     cufhe::CleanUp();
     std::printf(ok ? "ALL ANNIHILATE TESTS PASSED\n" : "ANNIHILATE TESTS FAILED\n");
     return ok ? 0 : 1;

@@ -160,3 +160,44 @@ def test_errors_without_key_and_bad_arguments(engine, secret):
     assert C.lib().cufhe_annihilate_key_switch_batch(1, d.ptr, N, d.ptr, 2 * N, 1, None) == ERR_ARG
     short = np.zeros(5, np.uint32)
     assert C.lib().cufhe_annihilate_key_load(1, short.ctypes.data, 5) == ERR_SIZE
+
+
+def test_lvl2_ntt_backend_bit_exact_vs_oracle(engine, secret):
+    """Level 2 on the Goldilocks NTT backend (the reference's USE_FFT=OFF
+    arithmetic): every output word equals the oracle's NTT mode, trivial
+    bodies come back exactly, the noiseless-key identity holds; loading an
+    FFT key afterwards switches the level back."""
+    N, T, w = LEVELS[2]
+    ahk = C.annihilate_key_gen(2, 36, secret[2])
+    C.annihilate_key_load(2, ahk, backend=C.BACKEND_NTT)
+    rng = np.random.default_rng(90)
+    m = _messages(2, rng, 3)
+    m[0] = 0
+    m[0, 0] = T(1) << T(w - 3)
+    ct = C.trlwe_encrypt_level(2, 81, secret[2], m)
+    got = _run(2, ct)
+    for i in range(3):
+        want = O.annihilate(2, O.NTT, ct[i], ahk)
+        print("ntt case", i, "mismatching words", int(np.sum(got[i] != want)), flush=True)
+        assert np.array_equal(got[i], want), i
+    triv = np.zeros((1, 2 * N), T)
+    triv[0, N] = T(1) << T(w - 3)
+    assert np.array_equal(_run(2, triv), triv)
+    C.annihilate_key_load(2, C.annihilate_key_gen(2, 37, secret[2], alpha=0.0),
+                          backend=C.BACKEND_NTT)
+    ph = C.trlwe_phase(2, secret[2], _run(2, ct))
+    want = np.zeros_like(ph)
+    want[:, 0] = C.trlwe_phase(2, secret[2], ct)[:, 0]
+    e = np.abs(_signed(ph - want, 2)).max()
+    print("ntt noiseless-key error 2^%.1f" % np.log2(e + 1), flush=True)
+    assert e < (1 << 33)  # key noise of the encryptions (alpha 2^-44) plus decomposition
+    C.annihilate_key_load(2, ahk)  # FFT again
+    fft = _run(2, ct[:1])
+    assert not np.array_equal(fft[0], got[0])
+    dph = _signed(C.trlwe_phase(2, secret[2], fft)[0] - C.trlwe_phase(2, secret[2], got[:1])[0], 2)
+    assert np.abs(dph).max() <= PHASE_TOL[2]
+
+
+def test_lvl1_ntt_backend_is_refused(engine, secret):
+    ahk = C.annihilate_key_gen(1, 38, secret[1])
+    assert C.lib().cufhe_annihilate_key_load_ex(1, ahk.ctypes.data, ahk.size, C.BACKEND_NTT) == -5
