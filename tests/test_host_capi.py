@@ -41,6 +41,28 @@ def test_version_and_error_paths_without_gpu_work():
     assert lib.cufhe_plan_create_lvl1(None, 0, None, 0, 0, ctypes.byref(ctypes.c_void_p())) == -3
 
 
+def test_annihilation_argument_errors_without_gpu_work():
+    """Checked before any device call: bad level, null pointers, strides below
+    the TRLWE size, an NTT key at level 1, a key of the wrong size (host
+    generation), an empty batch."""
+    lib = C.lib()
+    buf = ctypes.c_void_p(16)  # never dereferenced: every call below fails its checks first
+    assert lib.cufhe_annihilate_key_switch(3, buf, buf, None) == -1
+    assert lib.cufhe_annihilate_key_switch_batch(1, None, 2048, buf, 2048, 1, None) == -1
+    assert lib.cufhe_annihilate_key_switch_batch(1, buf, 2047, buf, 2048, 1, None) == -1
+    assert lib.cufhe_annihilate_key_switch_batch(2, buf, 4096, buf, 4095, 1, None) == -1
+    assert lib.cufhe_annihilate_key_switch_batch(1, buf, 2048, buf, 2048, 0, None) == 0
+    assert lib.cufhe_annihilate_key_load_ex(1, buf, 40960, 1) == -5
+    assert lib.cufhe_annihilate_key_load_ex(2, buf, 180224, 7) == -1
+    assert lib.cufhe_annihilate_key_delete(0) == -1
+    assert lib.tfhe_annihilate_key_words(1) == 10 * 2 * 2 * 1024
+    assert lib.tfhe_annihilate_key_words(2) == 11 * 4 * 2 * 2048
+    key = np.zeros(1024, np.uint32)
+    out = np.zeros(100, np.uint32)
+    assert lib.tfhe_annihilate_key_gen(1, 1, key.ctypes.data, 0.0, out.ctypes.data, 100) == -2
+    assert lib.tfhe_annihilate_key_gen(0, 1, key.ctypes.data, 0.0, out.ctypes.data, 100) == -1
+
+
 def test_parameter_sizes():
     lib = C.lib()
     assert lib.tfhe_tlwe_words(0) == 637
