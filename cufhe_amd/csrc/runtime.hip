@@ -1994,6 +1994,9 @@ int cufhe_annihilate_key_load_ex(int level, const void* ahk, size_t words, int b
     std::lock_guard<std::mutex> g(s.mu);
     const int rc = ensure_init();
     if (rc != CUFHE_OK) return rc;
+    // key switches still running on any stream may read the key memory that
+    // is rewritten or freed below
+    CUFHE_HIP_TRY(hipDeviceSynchronize());
     if (level == 1) return load_annihilate_key(s.ahk1, s.t1, ahk, words);
     if (backend == CUFHE_BACKEND_FFT) {
         const int r2 = load_annihilate_key(s.ahk2, s.t2, ahk, words);

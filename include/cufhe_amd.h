@@ -389,7 +389,9 @@ int cufhe_cmux_accumulate(int N, void* d_trlwe, const void* d_trgsw_spectra, uin
  * src/annihilate_gpu.cu:631-677 AnnihilateKeyPolynomialToDevice): the key in
  * the polynomial domain as tfhe_annihilate_key_gen writes it (nbit x l x 2
  * polynomials, tfhe_annihilate_key_words(level) words, host memory) is
- * converted to spectra on the device; a new load replaces the old key. */
+ * converted to spectra on the device; a new load replaces the old key (it
+ * first waits for all work on the device, so key switches in flight finish
+ * with the old key). */
 int cufhe_annihilate_key_load(int level, const void* ahk, size_t words);
 /* The same with the backend: CUFHE_BACKEND_FFT (both levels) or, at level 2,
  * CUFHE_BACKEND_NTT (the Goldilocks prime: the reference's USE_FFT=OFF
