@@ -127,6 +127,121 @@ __global__ __launch_bounds__(128) void k_annihilate_fft(typename P::T* out, size
     for (int i = threadIdx.x; i < 2 * N; i += 128) dst[i] = sc[i];
 }
 
+// lvl1 with ONE wave per TRLWE (the form the launcher uses at lvl1): the wave
+// decomposes tau_d(a) into both gadget digits at once, transforms the two
+// digit polynomials as an interleaved pair (fft_forward2), multiplies them
+// into both output components and inverse-transforms the two as a pair
+// (fft_inverse2). No workgroup barrier after the table copy: the waves of a
+// workgroup are independent TRLWEs. The transforms and the multiply-
+// accumulate order are those of k_annihilate_fft, so the output words are
+// identical.
+constexpr int kAh1Waves = 4;
+
+struct Ah1Shared {
+    static constexpr int N = lvl1param::n, H = N / 2, R = H / 64;
+    cplx twl[64];
+    cplx tws[64];
+    cplx tw64[64];
+    cplx cr[R];
+    uint32_t sc[kAh1Waves][2 * N];                       // per wave: its TRLWE (a, b)
+    cplx scratch[kAh1Waves][fft_stage::kScratchPad(H)];  // per wave: transform scratch
+};
+
+__global__ __launch_bounds__(64 * kAh1Waves) void k_annihilate_fft1w(uint32_t* out, size_t out_stride,
+                                                                    const uint32_t* in,
+                                                                    size_t in_stride, int batch,
+                                                                    const cplx* ahk, FFTTables gtb)
+{
+    using P = lvl1param;
+    using T = uint32_t;
+    constexpr int N = P::n, H = N / 2, R = H / 64, L = P::l;
+    static_assert(L == 2, "both digits as one transform pair");
+    __shared__ Ah1Shared sm;
+    const int t = threadIdx.x;
+    for (int i = t; i < 64; i += 64 * kAh1Waves) {
+        sm.twl[i] = gtb.twl[i];
+        sm.tws[i] = gtb.tws[i];
+        sm.tw64[i] = gtb.tw64[i];
+    }
+    for (int i = t; i < R; i += 64 * kAh1Waves) sm.cr[i] = gtb.cr[i];
+    __syncthreads();
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int lane = t & 63;
+    const size_t job = static_cast<size_t>(blockIdx.x) * kAh1Waves + wave;
+    if (job >= static_cast<size_t>(batch)) return;  // no workgroup barrier below
+    const FFTTables tb{sm.twl, sm.tw64, sm.cr, sm.tws};
+    T* const sc = sm.sc[wave];
+    cplx* const scratch = sm.scratch[wave];
+
+    const T* src = in + job * in_stride;
+#pragma unroll
+    for (int r = 0; r < 2 * N / 64; r++) sc[lane + 64 * r] = src[lane + 64 * r];
+    wave_lds_sync();
+
+    constexpr double scale = kTorusScale<T> / H;
+#pragma unroll 1
+    for (int bit = 0; bit < static_cast<int>(P::nbit); bit++) {
+        const uint32_t d = (2u << bit) + 1u;
+        uint32_t di = d;
+#pragma unroll
+        for (int k = 0; k < 4; k++) di *= 2u - d * di;
+        di &= 2u * N - 1u;
+        auto tau = [di](const T* p, int j) -> T {
+            const uint32_t u = (static_cast<uint32_t>(j) * di) & (2u * N - 1u);
+            const T v = p[u & (N - 1)] >> 1;
+            return u < static_cast<uint32_t>(N) ? v : static_cast<T>(0 - v);
+        };
+        cplx x[R], y[R];  // digit 0 and digit 1 of tau_d(a >> 1)
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int m = lane + 64 * r;
+            const T lo = tau(sc, m), hi = tau(sc, m + H);
+            x[r] = cplx{static_cast<double>(Gadget<P>::digit(lo, 0)),
+                        static_cast<double>(Gadget<P>::digit(hi, 0))};
+            y[r] = cplx{static_cast<double>(Gadget<P>::digit(lo, 1)),
+                        static_cast<double>(Gadget<P>::digit(hi, 1))};
+        }
+        fft_forward2<H>(x, y, scratch, tb);
+        // key [bit][row q][component o][H]: acc_o = D0 K(0, o) + D1 K(1, o)
+        const cplx* key = ahk + static_cast<size_t>(bit) * L * 2 * H + lane;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            cplx a0 = cplx{0.0, 0.0}, a1 = cplx{0.0, 0.0};
+            cmac(a0, x[r], key[(0 * 2 + 0) * H + 64 * r]);
+            cmac(a0, y[r], key[(1 * 2 + 0) * H + 64 * r]);
+            cmac(a1, x[r], key[(0 * 2 + 1) * H + 64 * r]);
+            cmac(a1, y[r], key[(1 * 2 + 1) * H + 64 * r]);
+            x[r] = a0;
+            y[r] = a1;
+        }
+        fft_inverse2<H>(x, y, scratch, tb);
+        T na[2 * R], nb[2 * R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int m = lane + 64 * r;
+            na[r] = static_cast<T>((sc[m] >> 1) - double_to_torus<T>(x[r].x * scale));
+            na[R + r] = static_cast<T>((sc[m + H] >> 1) - double_to_torus<T>(x[r].y * scale));
+            nb[r] = static_cast<T>((sc[N + m] >> 1) + tau(sc + N, m) -
+                                   double_to_torus<T>(y[r].x * scale));
+            nb[R + r] = static_cast<T>((sc[N + m + H] >> 1) + tau(sc + N, m + H) -
+                                       double_to_torus<T>(y[r].y * scale));
+        }
+        wave_lds_sync();  // every read of the round before the first write
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int m = lane + 64 * r;
+            sc[m] = na[r];
+            sc[m + H] = na[R + r];
+            sc[N + m] = nb[r];
+            sc[N + m + H] = nb[R + r];
+        }
+        wave_lds_sync();
+    }
+    T* dst = out + job * out_stride;
+#pragma unroll
+    for (int r = 0; r < 2 * N / 64; r++) dst[lane + 64 * r] = sc[lane + 64 * r];
+}
+
 __global__ __launch_bounds__(128) void k_annihilate_ntt64(uint64_t* out, size_t out_stride,
                                                           const uint64_t* in, size_t in_stride,
                                                           const uint64_t* ahk, Ntt64Tables tb)
@@ -218,6 +333,12 @@ void launch_annihilate_fft(typename P::T* out, size_t out_stride, const typename
                            hipStream_t st)
 {
     if (batch <= 0) return;
+    if constexpr (sizeof(typename P::T) == 4) {
+        hipLaunchKernelGGL(k_annihilate_fft1w, dim3((batch + kAh1Waves - 1) / kAh1Waves),
+                           dim3(64 * kAh1Waves), 0, st, out, out_stride, in, in_stride, batch, ahk,
+                           K.fft);
+        return;
+    }
     hipLaunchKernelGGL((k_annihilate_fft<P>), dim3(batch), dim3(128), 0, st, out, out_stride, in,
                        in_stride, ahk, K.fft);
 }
