@@ -86,14 +86,22 @@ __global__ __launch_bounds__(128) void k_annihilate_fft(typename P::T* out, size
             fft_forward<H>(z, spec[wave], tb);
 #pragma unroll
             for (int r = 0; r < R; r++) lds_store(&spec[wave][r * 64 + lane], z[r]);
+            // the key values of this wave's component for the pair's first
+            // digit, loaded while the pair meets at the barrier
+            const cplx* key0 =
+                ahk + ((static_cast<size_t>(bit) * L + 2 * pr) * 2 + wave) * H + lane;
+#pragma unroll
+            for (int r = 0; r < R; r++) z[r] = key0[r * 64];
             __syncthreads();  // both digit spectra of the pair written
 #pragma unroll
-            for (int w2 = 0; w2 < 2; w2++) {
-                const cplx* key =
-                    ahk + ((static_cast<size_t>(bit) * L + 2 * pr + w2) * 2 + wave) * H + lane;
+            for (int r = 0; r < R; r++) cmac(acc[r], lds_load(&spec[0][r * 64 + lane]), z[r]);
+            {
+                const cplx* key1 = key0 + 2 * H;
+#pragma unroll
+                for (int r = 0; r < R; r++) z[r] = key1[r * 64];
 #pragma unroll
                 for (int r = 0; r < R; r++)
-                    cmac(acc[r], lds_load(&spec[w2][r * 64 + lane]), key[r * 64]);
+                    cmac(acc[r], lds_load(&spec[1][r * 64 + lane]), z[r]);
             }
             __syncthreads();  // both waves done reading before the next pair's transforms
         }
@@ -291,16 +299,23 @@ __global__ __launch_bounds__(128) void k_annihilate_ntt64(uint64_t* out, size_t 
             ntt64_forward(x, buf[wave], tb, lane);
 #pragma unroll
             for (int r = 0; r < 32; r++) buf[wave][ntt64_addr(32 * lane + r)] = x[r];
+            // key word 64 r + lane holds transform position 32 lane + r; the
+            // pair's first digit's words are loaded while the pair meets at
+            // the barrier
+            const uint64_t* key0 =
+                ahk + ((static_cast<size_t>(bit) * L + 2 * pr) * 2 + wave) * N + lane;
+#pragma unroll
+            for (int r = 0; r < 32; r++) x[r] = key0[64 * r];
             __syncthreads();  // both digit spectra of the pair written
 #pragma unroll
             for (int w2 = 0; w2 < 2; w2++) {
-                // key word 64 r + lane holds transform position 32 lane + r
-                const uint64_t* key =
-                    ahk + ((static_cast<size_t>(bit) * L + 2 * pr + w2) * 2 + wave) * N + lane;
+                if (w2 == 1) {
+#pragma unroll
+                    for (int r = 0; r < 32; r++) x[r] = key0[2 * N + 64 * r];
+                }
                 const uint64_t* spec = buf[w2] + 33 * lane;
 #pragma unroll
-                for (int r = 0; r < 32; r++)
-                    sp[r] = gold::add(sp[r], gold::mul(spec[r], key[64 * r]));
+                for (int r = 0; r < 32; r++) sp[r] = gold::add(sp[r], gold::mul(spec[r], x[r]));
             }
             __syncthreads();  // both waves done reading before the next pair's transforms
         }
