@@ -295,12 +295,27 @@ __device__ __forceinline__ void br_step_ntt64(AccStore<lvl2param>& acc, int j, u
 #pragma unroll
             for (int r = 0; r < 32; r++) my_buf[ntt64_addr(32 * lane + r)] = x[r];
         }
+        // key column j of row (j, q) for this wave's own digit spectrum and
+        // of row (1 - j, q) for the partner's; word 64 r + lane holds position p
+        const uint64_t* const kown =
+            GHOST ? nullptr : trgsw0 + static_cast<size_t>((j * L + q) * 2 + j) * N;
+        const uint64_t* const koth =
+            GHOST ? nullptr : trgsw0 + static_cast<size_t>(((1 - j) * L + q) * 2 + j) * N;
+        // the first group's key words are loaded while the pair meets at the
+        // barrier (their L2 latency overlaps the wait)
+        constexpr int KM = KB ? 3 : 1;
+        uint64_t kpre[4][2][KM];
+        if constexpr (!GHOST) {
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int e = 0; e < KM; e++) {
+                    kpre[r][0][e] = kown[e * kTrgswNtt64 + 64 * r + lane];
+                    kpre[r][1][e] = koth[e * kTrgswNtt64 + 64 * r + lane];
+                }
+        }
         __syncthreads();  // both spectra of digit q published
         if constexpr (!GHOST) {
-            // key column j of row (j, q) for this wave's own digit spectrum and
-            // of row (1 - j, q) for the partner's; word 64 r + lane holds position p
-            const uint64_t* const kown = trgsw0 + static_cast<size_t>((j * L + q) * 2 + j) * N;
-            const uint64_t* const koth = trgsw0 + static_cast<size_t>(((1 - j) * L + q) * 2 + j) * N;
             uint32_t b0 = 0, b1 = 0, b01 = 0;
             uint64_t one = 0;
             if constexpr (KB) {
@@ -319,6 +334,13 @@ __device__ __forceinline__ void br_step_ntt64(AccStore<lvl2param>& acc, int j, u
                     constexpr int r = 4 * decltype(gc)::value + decltype(rc)::value;
                     constexpr uint32_t br5 = __builtin_bitreverse32(static_cast<uint32_t>(r)) >> 27;
                     const uint64_t d_own = own[r], d_oth = oth[r];
+                    // key word e of the own / partner row at this register
+                    auto kw = [&](int side, int e) -> uint64_t {
+                        if constexpr (r < 4)
+                            return kpre[r][side][e];
+                        else
+                            return (side == 0 ? kown : koth)[e * kTrgswNtt64 + 64 * r + lane];
+                    };
                     uint64_t k_own, k_oth;
                     if constexpr (KB) {
                         const uint64_t* const pw = K.tb.pw;
@@ -326,16 +348,16 @@ __device__ __forceinline__ void br_step_ntt64(AccStore<lvl2param>& acc, int j, u
                         const uint64_t xa1 = gold::sub(pw[(b1 + 128u * br5 * a1) & (2 * N - 1)], 1);
                         const uint64_t xa01 = gold::sub(pw[(b01 + 128u * br5 * a01) & (2 * N - 1)], 1);
                         // one + BK2 X(a1) + BK1 X(a0) + BK0 X(a0 + a1)
-                        k_own = gold::add(one, gold::mul(kown[2 * kTrgswNtt64 + 64 * r + lane], xa1));
-                        k_own = gold::add(k_own, gold::mul(kown[kTrgswNtt64 + 64 * r + lane], xa0));
-                        k_own = gold::add(k_own, gold::mul(kown[64 * r + lane], xa01));
-                        k_oth = gold::mul(koth[2 * kTrgswNtt64 + 64 * r + lane], xa1);
-                        k_oth = gold::add(k_oth, gold::mul(koth[kTrgswNtt64 + 64 * r + lane], xa0));
-                        k_oth = gold::add(k_oth, gold::mul(koth[64 * r + lane], xa01));
+                        k_own = gold::add(one, gold::mul(kw(0, 2), xa1));
+                        k_own = gold::add(k_own, gold::mul(kw(0, 1), xa0));
+                        k_own = gold::add(k_own, gold::mul(kw(0, 0), xa01));
+                        k_oth = gold::mul(kw(1, 2), xa1);
+                        k_oth = gold::add(k_oth, gold::mul(kw(1, 1), xa0));
+                        k_oth = gold::add(k_oth, gold::mul(kw(1, 0), xa01));
                     }
                     else {
-                        k_own = kown[64 * r + lane];
-                        k_oth = koth[64 * r + lane];
+                        k_own = kw(0, 0);
+                        k_oth = kw(1, 0);
                     }
                     sp[r] = gold::add(sp[r], gold::add(gold::mul(d_own, k_own), gold::mul(d_oth, k_oth)));
                 });
