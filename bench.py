@@ -33,17 +33,19 @@ sys.path.insert(0, ROOT)
 
 # The plain per-gate form runs one single-workgroup gate per launch over 64
 # streams; HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (default
-# 4), so at most 4 such gates run at once. 16 queues measured 3.4x the plain
-# form's throughput (987 -> 3,320 gates/s; 32 measured 1,637); the batched
-# path uses one stream and is unaffected. Set before HIP initialises (no HIP
-# call happens before main); --hw-queues 0 keeps the environment's value.
+# 4), so at most 4 such gates run at once. A sweep over 8-32 queues and 64 or
+# 128 streams peaked at 20 queues and 128 streams (4,710 gates/s against 987
+# with 4 queues; 16: 3,689; 24: 4,417 over 64 streams but 2,842 over 128; 32:
+# 1,489-1,871); the batched path uses one stream and is unaffected. Set
+# before HIP initialises (no HIP call happens before main); --hw-queues 0
+# keeps the environment's value.
 def _hw_queues(argv):
     for i, a in enumerate(argv):
         if a == "--hw-queues" and i + 1 < len(argv):
             return argv[i + 1]
         if a.startswith("--hw-queues="):
             return a.split("=", 1)[1]
-    return "16"
+    return "20"
 
 
 if _hw_queues(sys.argv) != "0":
@@ -823,7 +825,7 @@ def main():
     ap.add_argument("--latency-layers", type=int, default=200,
                     help="eager layers (timed steps included) behind p50/p99_layer_ms; the "
                          "extra layers run after the timed region")
-    ap.add_argument("--hw-queues", type=int, default=16,
+    ap.add_argument("--hw-queues", type=int, default=20,
                     help="GPU_MAX_HW_QUEUES for this process (1-32; 0 keeps the environment's)")
     ap.add_argument("--lvl2-backend", choices=["fft", "ntt"], default="fft",
                     help="lvl2mix: blind rotation on the FP64 FFT or the Goldilocks NTT backend")
@@ -833,7 +835,7 @@ def main():
                     help="bootstraps per workgroup; 0 = autotune over {1, 2, 4, 6, 8} "
                          "(6: key shared through LDS, FFT key bundle only)")
     ap.add_argument("--plain-sample", type=int, default=256)
-    ap.add_argument("--plain-streams", type=int, default=64)
+    ap.add_argument("--plain-streams", type=int, default=128)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
