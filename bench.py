@@ -710,6 +710,74 @@ def workload_annihilate(args):
     return res
 
 
+# ------------------------------------------------------------------ AES circuit
+def workload_aes(args):
+    """AES-128 decryption of args.aes_blocks blocks (each its own key) as gate
+    circuits on the serving loop, all submitted at once."""
+    import numpy as np
+
+    import cufhe_amd as C
+    from cufhe_amd import aes as A
+
+    log("keys (seed 0)")
+    k0, k1, _ = C.secret_key_gen(0)
+    bk = C.bk_gen(1, True, 0, k0, k1)
+    ksk = C.ksk_gen(1, 0, k1, k0)
+    C.check(C.lib().cufhe_initialize(), "initialize")
+    C.initialize_lvl01(bk, ksk)
+    circ = A.decrypt_circuit()
+    nb = args.aes_blocks
+    rng = np.random.default_rng(args.seed)
+    keys = [bytes(rng.integers(0, 256, 16, dtype=np.uint8).tolist()) for _ in range(nb)]
+    cts = [bytes(rng.integers(0, 256, 16, dtype=np.uint8).tolist()) for _ in range(nb)]
+    expect = None
+    if not args.no_cpu:
+        from oracle import aes_ref as R  # the expected plaintexts (checker only)
+        expect = [R.decrypt_block(k, c) for k, c in zip(keys, cts)]
+    ins = []
+    for i in range(nb):
+        # the key owner expands the key (FIPS-197 5.2) before encrypting it
+        rk = A.round_keys(keys[i])
+        bits = A.input_bits(cts[i], rk)
+        ins.append(np.concatenate([C.encrypt_bits(1, 4000 + i, k1, bits[:-2]),
+                                   A.trivial_bits(bits[-2:])]))
+    wires = circ.n_inputs + circ.n_gates
+    srv = C.Server(pool_bytes=(nb + 1) * wires * C.TLWE1_WORDS * 4,
+                   max_batch_bootstraps=args.aes_batch)
+    try:
+        warm = srv.submit(circ, ins[0])
+        srv.result(warm)
+        s0 = srv.stats()
+        C.check(C.lib().cufhe_synchronize(), "synchronize")
+        t0 = time.perf_counter()
+        rids = [srv.submit(circ, x) for x in ins]
+        outs = [srv.result(r) for r in rids]
+        total = time.perf_counter() - t0
+        lat = [srv.info(r)["done"] - srv.info(r)["submit"] for r in rids]
+        s1 = srv.stats()
+    finally:
+        srv.destroy()
+    got = [A.from_bits(C.decrypt_bits(1, k1, o)) for o in outs]
+    wrong = None if expect is None else int(sum(g != e for g, e in zip(got, expect)))
+    nbs = s1["bootstraps"] - s0["bootstraps"]
+    busy = s1["busy_s"] - s0["busy_s"]
+    log("aes: %d blocks in %.2f s (%d rounds, %d bootstraps, GPU busy %.2f s), %s wrong"
+        % (nb, total, s1["rounds"] - s0["rounds"], nbs, busy, wrong))
+    res = base_result(args, nb / total, total * 1e3, "f64", {
+        "workload": "AES-128 decryption as a Boolean circuit of lvl1 gates (%d gates, %d "
+                    "bootstraps, %d layers per block), %d blocks with their own keys "
+                    "submitted together to the serving loop" % (circ.n_gates, circ.bootstraps,
+                                                                 circ.depth, nb),
+        "blocks": nb, "parallelism": "single GPU"})
+    res["metric"] = "AES-128 decryption blocks/s (gate circuit, serving loop)"
+    res["unit"] = "blocks/s"
+    res["steps"] = 1
+    res.update({"bootstraps_per_s": round(nbs / total, 1), "rounds": s1["rounds"] - s0["rounds"],
+                "gpu_busy_s": round(busy, 3), "block_latency_p50_s": round(pct(lat, 50), 3),
+                "wrong_blocks": wrong, "cpu_baseline": None, "roofline": None})
+    return res
+
+
 # ------------------------------------------------------------------ serving
 def workload_serving(args):
     import threading
@@ -812,7 +880,7 @@ def workload_serving(args):
 
 
 WORKLOADS = {"kb4096": workload_kb4096, "lvl2mix": workload_lvl2mix, "nttbb": workload_nttbb,
-             "serving": workload_serving, "annihilate": workload_annihilate}
+             "serving": workload_serving, "annihilate": workload_annihilate, "aes": workload_aes}
 
 
 def main():
@@ -835,6 +903,9 @@ def main():
                     help="bootstraps per workgroup; 0 = autotune over {1, 2, 4, 6, 8} "
                          "(6: key shared through LDS, FFT key bundle only)")
     ap.add_argument("--plain-sample", type=int, default=256)
+    ap.add_argument("--aes-blocks", type=int, default=8, help="aes: blocks submitted together")
+    ap.add_argument("--aes-batch", type=int, default=16384,
+                    help="aes: the server's bootstraps per round")
     ap.add_argument("--plain-streams", type=int, default=128)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
