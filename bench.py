@@ -778,6 +778,71 @@ def workload_aes(args):
     return res
 
 
+# ------------------------------------------------------------------ ASCON circuit
+def workload_ascon(args):
+    """ASCON-XOF (args.ascon_in_bytes -> args.ascon_out_bytes) of
+    args.ascon_messages messages as gate circuits on the serving loop, all
+    submitted at once."""
+    import numpy as np
+
+    import cufhe_amd as C
+    from cufhe_amd import aes as A
+    from cufhe_amd import ascon as S
+
+    log("keys (seed 0)")
+    k0, k1, _ = C.secret_key_gen(0)
+    bk = C.bk_gen(1, True, 0, k0, k1)
+    ksk = C.ksk_gen(1, 0, k1, k0)
+    C.check(C.lib().cufhe_initialize(), "initialize")
+    C.initialize_lvl01(bk, ksk)
+    circ = S.xof_circuit(args.ascon_in_bytes, args.ascon_out_bytes)
+    nm = args.ascon_messages
+    rng = np.random.default_rng(args.seed)
+    msgs = [bytes(rng.integers(0, 256, args.ascon_in_bytes, dtype=np.uint8).tolist())
+            for _ in range(nm)]
+    expect = None
+    if not args.no_cpu:
+        from oracle import ascon_ref as R  # the expected outputs (checker only)
+        expect = [R.ascon_xof(m, args.ascon_out_bytes) for m in msgs]
+    ins = [np.concatenate([C.encrypt_bits(1, 5000 + i, k1, A.bits(m)), A.trivial_bits([0, 1])])
+           for i, m in enumerate(msgs)]
+    wires = circ.n_inputs + circ.n_gates
+    srv = C.Server(pool_bytes=(nm + 1) * wires * C.TLWE1_WORDS * 4,
+                   max_batch_bootstraps=args.aes_batch)
+    try:
+        srv.result(srv.submit(circ, ins[0]))
+        s0 = srv.stats()
+        C.check(C.lib().cufhe_synchronize(), "synchronize")
+        t0 = time.perf_counter()
+        rids = [srv.submit(circ, x) for x in ins]
+        outs = [srv.result(r) for r in rids]
+        total = time.perf_counter() - t0
+        lat = [srv.info(r)["done"] - srv.info(r)["submit"] for r in rids]
+        s1 = srv.stats()
+    finally:
+        srv.destroy()
+    got = [A.from_bits(C.decrypt_bits(1, k1, o)) for o in outs]
+    wrong = None if expect is None else int(sum(g != e for g, e in zip(got, expect)))
+    nbs = s1["bootstraps"] - s0["bootstraps"]
+    log("ascon: %d messages in %.2f s (%d rounds, %d bootstraps), %s wrong"
+        % (nm, total, s1["rounds"] - s0["rounds"], nbs, wrong))
+    res = base_result(args, nm / total, total * 1e3, "f64", {
+        "workload": "ASCON-XOF (%d-byte messages, %d-byte outputs) as a Boolean circuit of "
+                    "lvl1 gates (%d gates, %d bootstraps, %d layers per message), %d messages "
+                    "submitted together to the serving loop"
+                    % (args.ascon_in_bytes, args.ascon_out_bytes, circ.n_gates,
+                       circ.bootstraps, circ.depth, nm),
+        "messages": nm, "parallelism": "single GPU"})
+    res["metric"] = "ASCON-XOF messages/s (gate circuit, serving loop)"
+    res["unit"] = "messages/s"
+    res["steps"] = 1
+    res.update({"bootstraps_per_s": round(nbs / total, 1), "rounds": s1["rounds"] - s0["rounds"],
+                "gpu_busy_s": round(s1["busy_s"] - s0["busy_s"], 3),
+                "message_latency_p50_s": round(pct(lat, 50), 3), "wrong_outputs": wrong,
+                "cpu_baseline": None, "roofline": None})
+    return res
+
+
 # ------------------------------------------------------------------ serving
 def workload_serving(args):
     import threading
@@ -880,7 +945,8 @@ def workload_serving(args):
 
 
 WORKLOADS = {"kb4096": workload_kb4096, "lvl2mix": workload_lvl2mix, "nttbb": workload_nttbb,
-             "serving": workload_serving, "annihilate": workload_annihilate, "aes": workload_aes}
+             "serving": workload_serving, "annihilate": workload_annihilate, "aes": workload_aes,
+             "ascon": workload_ascon}
 
 
 def main():
@@ -905,7 +971,10 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--aes-blocks", type=int, default=8, help="aes: blocks submitted together")
     ap.add_argument("--aes-batch", type=int, default=16384,
-                    help="aes: the server's bootstraps per round")
+                    help="aes, ascon: the server's bootstraps per round")
+    ap.add_argument("--ascon-messages", type=int, default=32)
+    ap.add_argument("--ascon-in-bytes", type=int, default=16)
+    ap.add_argument("--ascon-out-bytes", type=int, default=32)
     ap.add_argument("--plain-streams", type=int, default=128)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
