@@ -28,7 +28,7 @@ least significant bit first, the reference's byte-major order), then one
 encryption of 0 and one of 1 (aes.trivial_bits builds them); outputs in the
 same bit order.
 """
-from .aes import _Builder
+from .aes import _Builder, _Lits
 from .circuits import Circuit  # noqa: F401  (the return type)
 
 M64 = (1 << 64) - 1
@@ -58,48 +58,6 @@ def _p12_plain(x):
         for i, (a, b) in enumerate(_ROT):
             x[i] ^= _rotr(x[i], a) ^ _rotr(x[i], b)
     return x
-
-
-class _Lits:
-    """Literals (wire, negated) over a builder; (c0, False) is 0, (c0, True) 1."""
-
-    def __init__(self, b, c0, c1):
-        self.b, self.c0, self.c1 = b, c0, c1
-
-    def const(self, v):
-        return (self.c0, bool(v))
-
-    def xor(self, p, q):
-        (a, na), (b, nb) = p, q
-        if a == self.c0:
-            return (b, nb ^ na)
-        if b == self.c0:
-            return (a, na ^ nb)
-        if a == b:
-            return (self.c0, na ^ nb)
-        return (self.b.gate("XOR", a, b), na ^ nb)
-
-    def neg(self, p):
-        return (p[0], not p[1])
-
-    def and_(self, p, q):
-        (a, na), (b, nb) = p, q
-        if a == self.c0:
-            return q if na else self.const(0)
-        if b == self.c0:
-            return p if nb else self.const(0)
-        if a == b:
-            return p if na == nb else self.const(0)
-        op = {(False, False): "AND", (True, False): "ANDNY", (False, True): "ANDYN",
-              (True, True): "NOR"}[(na, nb)]
-        return (self.b.gate(op, a, b), False)
-
-    def wire(self, p):
-        """A wire carrying the literal (a NOT gate only for a negated one)."""
-        a, na = p
-        if a == self.c0:
-            return self.c1 if na else self.c0
-        return self.b.gate("NOT", a) if na else a
 
 
 def _permutation(L, x):

@@ -28,6 +28,42 @@ def test_circuit_table_is_the_inverse_sbox():
     assert A.INV_SBOX == R.INV_SBOX
 
 
+def test_tower_field_isomorphism_and_karatsuba():
+    """The map into GF((2^4)^2) is a field isomorphism (all 65,536 products),
+    and the Karatsuba products give every GF(2^4) product."""
+    to = A._to_tower
+    assert sorted(to(x) for x in range(256)) == list(range(256))
+    for x in range(256):
+        tx = to(x)
+        for y in range(256):
+            assert to(R.gf_mul(x, y)) == A._mul8(tx, to(y))
+    for a in range(16):
+        for b in range(16):
+            prods = [A._parity(a & m) & A._parity(b & m) for m in A._KARA]
+            got = sum((sum(prods[t] for t in A._KARA_SUB[k]) & 1) << k for k in range(4))
+            assert got == A._mul4(a, b)
+
+
+def test_inv_sbox_circuit_exhaustive():
+    """The S-box sub-circuit of the decryption over all 256 inputs."""
+    c = A.inv_sbox_circuit()
+    for y in range(256):
+        out = c.evaluate(np.concatenate([A.bits([y]), [0, 1]]))
+        assert A.from_bits(out)[0] == R.INV_SBOX[y], y
+
+
+def test_inv_sbox_circuit_catches_a_wrong_isomorphism(monkeypatch):
+    """A basis image off by one bit (a plausible slip) breaks the S-box."""
+    monkeypatch.setattr(A, "_PLAN", None)
+    bad = list(A._BASIS)
+    bad[3] ^= 1
+    monkeypatch.setattr(A, "_BASIS", bad)
+    monkeypatch.setattr(A, "_FROM_TOWER", {A._to_tower(x): x for x in range(256)})
+    c = A.inv_sbox_circuit()
+    got = [A.from_bits(c.evaluate(np.concatenate([A.bits([y]), [0, 1]])))[0] for y in range(256)]
+    assert got != R.INV_SBOX
+
+
 def test_host_key_expansion_matches_the_oracle():
     k = bytes.fromhex("2b7e151628aed2a6abf7158809cf4f3c")
     assert A.round_keys(k) == R.expand_key(k)
