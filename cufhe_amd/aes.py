@@ -373,22 +373,27 @@ def _plan():
     if _PLAN is not None:
         return _PLAN
     n = len(_KARA)
-    # linear forms of z = tower(AffineInv(y)) ^ tower(AffineInv(0x63)),
-    # z = (a << 4) | b: the a- and b-sums of the Karatsuba products and
-    # lam a^2 + b^2
-    ell = _rows(lambda z: _mul4(_mul4(z >> 4, z >> 4), _LAM) ^ _mul4(z & 15, z & 15), 8, 4)
-    forms_z = [m << 4 for m in _KARA] + list(_KARA) + ell
+    # z = tower(AffineInv(y)) ^ tower(AffineInv(0x63)) of the S-box input
+    # y, and from z = (a << 4) | b the linear forms the products need: the
+    # a- and b-sums of the Karatsuba products and lam a^2 + b^2
     m_in = _rows(lambda y: _to_tower(_AFFINE_INV[y]), 8, 8)
     c_in = _to_tower(_AFFINE_INV[0x63])
-    rows_y, flips = [], []
-    for m in forms_z:
-        r = 0
+    flips = [(c_in >> k) & 1 for k in range(8)]
+    to_z = _xor_network(m_in, 8)
+    ell = _rows(lambda z: _mul4(_mul4(z >> 4, z >> 4), _LAM) ^ _mul4(z & 15, z & 15), 8, 4)
+    forms = _xor_network([m << 4 for m in _KARA] + list(_KARA) + ell, 8)
+    # InvMixColumns followed by the map to z of each byte, as one 32 x 32
+    # network per column (cheaper than the two apart, one layer shallower)
+    mix = [sum(1 << i for i in row) for row in _inv_mix_matrix()]
+    mix_z = []
+    for byte in range(4):
         for k in range(8):
-            if (m >> k) & 1:
-                r ^= m_in[k]
-        rows_y.append(r)
-        flips.append(_parity(m & c_in))
-    forms = _xor_network(rows_y, 8)
+            r = 0
+            for i in range(8):
+                if (m_in[k] >> i) & 1:
+                    r ^= mix[8 * byte + i]
+            mix_z.append(r)
+    mix_z = _xor_network(mix_z, 32)
     # D's argument: ell_k ^ (a b)_k over [ell 0..3, P 0..8]
     delta = _xor_network([(1 << k) | sum(1 << (4 + t) for t in _KARA_SUB[k]) for k in range(4)],
                          4 + n, in_level=[0] * 4 + [1] * n)
@@ -409,10 +414,6 @@ def _plan():
     late = [1 if bin(m).count("1") == 2 else 0 for m in _KARA]
     out = _xor_network([out_row(j) for j in range(8)], 2 * n + 8,
                        in_level=[1 + x for x in late] * 2 + [0] * 8)
-    mix_rows = []
-    for row in _inv_mix_matrix():
-        mix_rows.append(sum(1 << i for i in row))
-    mix = _xor_network(mix_rows, 32)
     # Shannon order for the 4-bit inverse: fewest bootstraps, then depth
     best = None
     for order in itertools.permutations(range(4)):
@@ -426,8 +427,8 @@ def _plan():
         key = (c.bootstraps, max(b.level))
         if best is None or key < best[0]:
             best = (key, order)
-    _PLAN = {"forms": forms, "flips": flips, "delta": delta, "out": out, "mix": mix,
-             "order": best[1]}
+    _PLAN = {"to_z": to_z, "flips": flips, "forms": forms, "mix_z": mix_z, "delta": delta,
+             "out": out, "order": best[1]}
     return _PLAN
 
 
@@ -540,8 +541,9 @@ def inv_sbox_circuit():
     plan = _plan()
     b = _Builder(10)
     L = _Lits(b, 8, 9)
-    forms = _run(L, plan["forms"], [(i, False) for i in range(8)], plan["flips"])
-    out = _run(L, plan["out"], _inv_sbox_bytes(L, plan, forms) + [L.const(0)] * 8)
+    z = _run(L, plan["to_z"], [(i, False) for i in range(8)], plan["flips"])
+    out = _run(L, plan["out"], _inv_sbox_bytes(L, plan, _run(L, plan["forms"], z)) +
+               [L.const(0)] * 8)
     return b.circuit([L.wire(v) for v in out])
 
 
@@ -558,19 +560,20 @@ def decrypt_circuit():
     L = _Lits(b, CONST0, CONST1)
     key = lambda r, i: (128 + 128 * r + i, False)  # noqa: E731
     state = [L.xor((i, False), key(10, i)) for i in range(128)]
-    byte_forms = lambda x: _run(L, plan["forms"], x, plan["flips"])  # noqa: E731
-    forms = [byte_forms(x) for x in _inv_shift_rows([state[8 * i:8 * i + 8] for i in range(16)])]
+    byte_forms = lambda z: _run(L, plan["forms"], z)  # noqa: E731
+    shifted = _inv_shift_rows([state[8 * i:8 * i + 8] for i in range(16)])
+    forms = [byte_forms(_run(L, plan["to_z"], x, plan["flips"])) for x in shifted]
     for rnd in range(9, -1, -1):
         out = [_run(L, plan["out"], _inv_sbox_bytes(L, plan, forms[i]) +
                     [key(rnd, 8 * i + j) for j in range(8)]) for i in range(16)]
         if rnd == 0:
             state = sum(out, [])
             break
-        mixed = []
+        zs = []
         for c in range(4):
-            col = _run(L, plan["mix"], sum(out[4 * c:4 * c + 4], []))
-            mixed += [col[8 * r:8 * r + 8] for r in range(4)]
-        forms = [byte_forms(x) for x in _inv_shift_rows(mixed)]
+            col = _run(L, plan["mix_z"], sum(out[4 * c:4 * c + 4], []), plan["flips"] * 4)
+            zs += [col[8 * r:8 * r + 8] for r in range(4)]
+        forms = [byte_forms(z) for z in _inv_shift_rows(zs)]
     _CIRCUIT = b.circuit([L.wire(v) for v in state])
     return _CIRCUIT
 
