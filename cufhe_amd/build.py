@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libcufhe_amd.so")
 
 HIP_SOURCES = ["kernels_fft.hip", "kernels_ntt.hip", "kernels_ntt64.hip", "kernels_annihilate.hip",
                "ks_gemm.hip", "runtime.hip", "server.hip"]
-CXX_SOURCES = ["tfhe_host.cpp"]
+CXX_SOURCES = ["tfhe_host.cpp", "circuits_host.cpp"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 HIP_FLAGS = [

@@ -5,6 +5,8 @@ Wires [0, n_inputs) are the inputs; gate g writes wire n_inputs + g. Layer l
 holds `width` gates whose operands are wires of layer l - 1 (the inputs for
 l = 0); the outputs are the last layer's wires.
 """
+import ctypes
+
 import numpy as np
 
 from . import GATES, TRUTH
@@ -77,3 +79,37 @@ def random_circuit(rng, width, depth, mux_frac=0.1):
     layer_end = np.arange(1, depth + 1, dtype=np.int32) * width
     outputs = np.arange(width + (depth - 1) * width, width + depth * width, dtype=np.uint32)
     return Circuit(width, ops, in0, in1, inc, layer_end, outputs)
+
+
+class _Sizes(ctypes.Structure):
+    _fields_ = [("n_inputs", ctypes.c_int), ("n_gates", ctypes.c_int),
+                ("n_layers", ctypes.c_int), ("n_outputs", ctypes.c_int)]
+
+
+def _native(fn, *args):
+    """A circuit built by the library (include/cufhe_circuits.h): sizes
+    first, then the arrays."""
+    from . import check
+    sz = _Sizes()
+    check(fn(*args, ctypes.byref(sz), None, None, None), fn.__name__)
+    gates = np.zeros((sz.n_gates, 5), np.uint32)
+    layer_end = np.zeros(sz.n_layers, np.int32)
+    outputs = np.zeros(sz.n_outputs, np.uint32)
+    check(fn(*args, ctypes.byref(sz), gates.ctypes.data, layer_end.ctypes.data,
+             outputs.ctypes.data), fn.__name__)
+    return Circuit(sz.n_inputs, gates[:, 0].view(np.int32).astype(np.uint32), gates[:, 2],
+                   gates[:, 3], gates[:, 4], layer_end, outputs)
+
+
+def native_aes128_decrypt():
+    """AES-128 decryption as built by the library's C++ builder (the circuit
+    behind the C++ AESDec), in aes.py's wire layout."""
+    from . import lib
+    return _native(lib().cufhe_circuit_aes128_decrypt)
+
+
+def native_ascon_xof(in_bytes, out_bytes):
+    """ASCON-XOF as built by the library's C++ builder (behind the C++
+    ASCONXOF), in ascon.py's wire layout."""
+    from . import lib
+    return _native(lib().cufhe_circuit_ascon_xof, int(in_bytes), int(out_bytes))

@@ -57,3 +57,12 @@ def test_cpp_annihilate_api():
     test/test_annihilate_gpu.cu)."""
     p = _run("test_annihilate_gpu.bin")
     assert p.returncode == 0 and "ALL ANNIHILATE TESTS PASSED" in p.stdout
+
+
+def test_cpp_aes_ascon_api():
+    """AESDec and ASCONXOF through the C++ drop-ins (include/aes_gpu.hpp,
+    include/ascon_gpu.hpp): FIPS-197 Appendix B and C.1 blocks decrypted from
+    two host threads at once, the published empty-message Ascon-Xof digest
+    and the oracle's XOF of 00 .. 0f."""
+    p = _run("test_aes_ascon_gpu.bin")
+    assert p.returncode == 0 and "PASS" in p.stdout

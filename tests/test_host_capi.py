@@ -13,7 +13,7 @@ from conftest import ROOT, signed32
 
 def _declared_functions():
     names = set()
-    for h in ("cufhe_amd.h", "tfhe_host.h"):
+    for h in ("cufhe_amd.h", "tfhe_host.h", "cufhe_circuits.h"):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b((?:cufhe|tfhe)_\w+)\s*\(", text, re.M):
