@@ -150,6 +150,9 @@ def lib():
         "cufhe_annihilate_key_switch_batch": ([i32, vp, sz, vp, sz, sz, vp], i32),
         "cufhe_circuit_aes128_decrypt": ([vp, vp, vp, vp], i32),
         "cufhe_circuit_ascon_xof": ([i32, i32, vp, vp, vp, vp], i32),
+        "cufhe_circuit_ascon_absorb": ([i32, vp, vp, vp, vp], i32),
+        "cufhe_circuit_ascon_squeeze": ([i32, vp, vp, vp, vp], i32),
+        "cufhe_ascon_xof_initial_state": ([vp], i32),
         "cufhe_server_create": ([sz, i32, i32, ctypes.POINTER(vp)], i32),
         "cufhe_server_submit": ([vp, vp, vp, vp, ctypes.POINTER(u64)], i32),
         "cufhe_server_wait": ([vp, u64, i32], i32),

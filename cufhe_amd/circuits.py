@@ -113,3 +113,17 @@ def native_ascon_xof(in_bytes, out_bytes):
     ASCONXOF), in ascon.py's wire layout."""
     from . import lib
     return _native(lib().cufhe_circuit_ascon_xof, int(in_bytes), int(out_bytes))
+
+
+def native_ascon_absorb(in_bytes):
+    """The XOF's absorb over an encrypted state (cufhe_circuit_ascon_absorb):
+    inputs the 320 state bits, the message, 0 and 1; outputs the state."""
+    from . import lib
+    return _native(lib().cufhe_circuit_ascon_absorb, int(in_bytes))
+
+
+def native_ascon_squeeze(out_bytes):
+    """The XOF's squeeze (cufhe_circuit_ascon_squeeze): inputs the 320 state
+    bits, 0 and 1; outputs the output bits, then the state."""
+    from . import lib
+    return _native(lib().cufhe_circuit_ascon_squeeze, int(out_bytes))

@@ -30,6 +30,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <new>
 #include <numeric>
 #include <stdexcept>
 #include <utility>
@@ -703,6 +704,60 @@ void permutation(Lits& L, std::array<Word, 5>& x)
     }
 }
 
+int rate_pos(int j, int bit) { return 8 * (7 - j) + bit; }  // rate byte j, MSB first
+
+// Absorb of the reference's XOF (src/ascon_gpu.cu:1051-1077): whole 8-byte
+// blocks XORed into x0 and permuted, the rest XORed, the padding bit of
+// byte `left` flipped, p^12. Message bit i is wire msg0 + i.
+void absorb(Lits& L, std::array<Word, 5>& x, uint32_t msg0, int in_bytes)
+{
+    auto msg = [&](int i) { return Lit{msg0 + static_cast<uint32_t>(i), false}; };
+    int left = in_bytes, off = 0;
+    while (left >= 8) {
+        for (int j = 0; j < 8; j++)
+            for (int bit = 0; bit < 8; bit++)
+                x[0][rate_pos(j, bit)] = L.x(x[0][rate_pos(j, bit)], msg(8 * (off + j) + bit));
+        permutation(L, x);
+        off += 8;
+        left -= 8;
+    }
+    for (int j = 0; j < left; j++)
+        for (int bit = 0; bit < 8; bit++)
+            x[0][rate_pos(j, bit)] = L.x(x[0][rate_pos(j, bit)], msg(8 * (off + j) + bit));
+    x[0][rate_pos(left, 7)] = Lits::neg(x[0][rate_pos(left, 7)]);  // padding byte 0x80
+    permutation(L, x);
+}
+
+// Squeeze (src/ascon_gpu.cu:1080-1104): x0's bytes, p^12 between blocks and
+// none after the last.
+std::vector<uint32_t> squeeze(Lits& L, std::array<Word, 5>& x, int out_bytes)
+{
+    std::vector<uint32_t> outs;
+    for (int left = out_bytes;;) {
+        const int n = std::min(left, 8);
+        for (int j = 0; j < n; j++)
+            for (int bit = 0; bit < 8; bit++) outs.push_back(L.wire(x[0][rate_pos(j, bit)]));
+        left -= n;
+        if (left == 0) break;
+        permutation(L, x);
+    }
+    return outs;
+}
+
+std::array<Word, 5> state_inputs()
+{
+    std::array<Word, 5> x;
+    for (int i = 0; i < 5; i++)
+        for (int k = 0; k < 64; k++) x[i][k] = Lit{static_cast<uint32_t>(64 * i + k), false};
+    return x;
+}
+
+void append_state(Lits& L, const std::array<Word, 5>& x, std::vector<uint32_t>& outs)
+{
+    for (int i = 0; i < 5; i++)
+        for (int k = 0; k < 64; k++) outs.push_back(L.wire(x[i][k]));
+}
+
 Built build_ascon_xof(int in_bytes, int out_bytes)
 {
     const uint32_t c0 = 8 * in_bytes, c1 = c0 + 1;
@@ -712,31 +767,8 @@ Built build_ascon_xof(int in_bytes, int out_bytes)
     std::array<Word, 5> x;
     for (int i = 0; i < 5; i++)
         for (int k = 0; k < 64; k++) x[i][k] = L.konst((s0[i] >> k) & 1);
-    auto pos = [](int j, int bit) { return 8 * (7 - j) + bit; };  // rate byte j, MSB first
-    auto msg = [](int i) { return Lit{static_cast<uint32_t>(i), false}; };
-    int left = in_bytes, off = 0;
-    while (left >= 8) {
-        for (int j = 0; j < 8; j++)
-            for (int bit = 0; bit < 8; bit++)
-                x[0][pos(j, bit)] = L.x(x[0][pos(j, bit)], msg(8 * (off + j) + bit));
-        permutation(L, x);
-        off += 8;
-        left -= 8;
-    }
-    for (int j = 0; j < left; j++)
-        for (int bit = 0; bit < 8; bit++)
-            x[0][pos(j, bit)] = L.x(x[0][pos(j, bit)], msg(8 * (off + j) + bit));
-    x[0][pos(left, 7)] = Lits::neg(x[0][pos(left, 7)]);  // padding byte 0x80
-    permutation(L, x);
-    std::vector<uint32_t> outs;
-    for (left = out_bytes;;) {
-        const int n = std::min(left, 8);
-        for (int j = 0; j < n; j++)
-            for (int bit = 0; bit < 8; bit++) outs.push_back(L.wire(x[0][pos(j, bit)]));
-        left -= n;
-        if (left == 0) break;
-        permutation(L, x);
-    }
+    absorb(L, x, 0, in_bytes);
+    std::vector<uint32_t> outs = squeeze(L, x, out_bytes);
     if (b.gates.empty()) {
         // an empty message: the output is public; one COPY gate per distinct
         // constant wire gives the server a circuit with one layer
@@ -748,20 +780,51 @@ Built build_ascon_xof(int in_bytes, int out_bytes)
     return b.finish(outs);
 }
 
+// state (320) and message in, state out
+Built build_ascon_absorb(int in_bytes)
+{
+    const uint32_t c0 = 320 + 8 * in_bytes;
+    Builder b(static_cast<int>(c0) + 2);
+    Lits L{b, c0, c0 + 1};
+    std::array<Word, 5> x = state_inputs();
+    absorb(L, x, 320, in_bytes);
+    std::vector<uint32_t> outs;
+    append_state(L, x, outs);
+    return b.finish(outs);
+}
+
+// state in, output bytes and the state after the squeeze out
+Built build_ascon_squeeze(int out_bytes)
+{
+    const uint32_t c0 = 320;
+    Builder b(322);
+    Lits L{b, c0, c0 + 1};
+    std::array<Word, 5> x = state_inputs();
+    std::vector<uint32_t> outs = squeeze(L, x, out_bytes);
+    append_state(L, x, outs);
+    return b.finish(outs);
+}
+
 int deliver(const Built& c, cufhe_circuit_sizes* sizes, cufhe_gate_desc* gates, int* layer_end,
             uint32_t* outputs)
 {
     if (!sizes) return CUFHE_ERR_ARG;
     const bool fill = gates || layer_end || outputs;
     if (fill) {
-        if (!gates || !layer_end || !outputs) return CUFHE_ERR_ARG;
+        // a buffer may be NULL only where its array is empty (no gates)
+        if ((!gates && !c.gates.empty()) || (!layer_end && !c.layer_end.empty()) ||
+            (!outputs && !c.outputs.empty()))
+            return CUFHE_ERR_ARG;
         if (sizes->n_gates < static_cast<int>(c.gates.size()) ||
             sizes->n_layers < static_cast<int>(c.layer_end.size()) ||
             sizes->n_outputs < static_cast<int>(c.outputs.size()))
             return CUFHE_ERR_SIZE;
-        std::memcpy(gates, c.gates.data(), c.gates.size() * sizeof(cufhe_gate_desc));
-        std::memcpy(layer_end, c.layer_end.data(), c.layer_end.size() * sizeof(int));
-        std::memcpy(outputs, c.outputs.data(), c.outputs.size() * sizeof(uint32_t));
+        if (!c.gates.empty())
+            std::memcpy(gates, c.gates.data(), c.gates.size() * sizeof(cufhe_gate_desc));
+        if (!c.layer_end.empty())
+            std::memcpy(layer_end, c.layer_end.data(), c.layer_end.size() * sizeof(int));
+        if (!c.outputs.empty())
+            std::memcpy(outputs, c.outputs.data(), c.outputs.size() * sizeof(uint32_t));
     }
     sizes->n_inputs = c.n_inputs;
     sizes->n_gates = static_cast<int>(c.gates.size());
@@ -790,6 +853,36 @@ int cufhe_circuit_ascon_xof(int in_bytes, int out_bytes, cufhe_circuit_sizes* si
     try {
         const Built c = build_ascon_xof(in_bytes, out_bytes);
         return deliver(c, sizes, gates, layer_end, outputs);
+    } catch (const std::bad_alloc&) {
+        return CUFHE_ERR_NOMEM;
+    }
+}
+
+int cufhe_ascon_xof_initial_state(uint64_t* words)
+{
+    if (!words) return CUFHE_ERR_ARG;
+    const auto s0 = p12_plain({kXofIV, 0, 0, 0, 0});
+    std::copy(s0.begin(), s0.end(), words);
+    return CUFHE_OK;
+}
+
+int cufhe_circuit_ascon_absorb(int in_bytes, cufhe_circuit_sizes* sizes, cufhe_gate_desc* gates,
+                               int* layer_end, uint32_t* outputs)
+{
+    if (in_bytes < 0 || in_bytes > CUFHE_ASCON_MAX_BYTES) return CUFHE_ERR_ARG;
+    try {
+        return deliver(build_ascon_absorb(in_bytes), sizes, gates, layer_end, outputs);
+    } catch (const std::bad_alloc&) {
+        return CUFHE_ERR_NOMEM;
+    }
+}
+
+int cufhe_circuit_ascon_squeeze(int out_bytes, cufhe_circuit_sizes* sizes, cufhe_gate_desc* gates,
+                                int* layer_end, uint32_t* outputs)
+{
+    if (out_bytes <= 0 || out_bytes > CUFHE_ASCON_MAX_BYTES) return CUFHE_ERR_ARG;
+    try {
+        return deliver(build_ascon_squeeze(out_bytes), sizes, gates, layer_end, outputs);
     } catch (const std::bad_alloc&) {
         return CUFHE_ERR_NOMEM;
     }

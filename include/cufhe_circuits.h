@@ -47,6 +47,23 @@ int cufhe_circuit_aes128_decrypt(cufhe_circuit_sizes* sizes, cufhe_gate_desc* ga
 /* 0 <= in_bytes <= CUFHE_ASCON_MAX_BYTES, 0 < out_bytes <= CUFHE_ASCON_MAX_BYTES. */
 int cufhe_circuit_ascon_xof(int in_bytes, int out_bytes, cufhe_circuit_sizes* sizes,
                             cufhe_gate_desc* gates, int* layer_end, uint32_t* outputs);
+/* The XOF's two phases over an encrypted state (reference
+ * src/ascon_gpu.cu:1051-1104 DeviceASCONXOFAbsorb / Squeeze). State wire
+ * 64 i + k is bit k of word x_i (bit k = 2^k).
+ *   absorb:  inputs [0, 320) the state, [320, 320 + 8 in_bytes) the message,
+ *            then encryptions of 0 and 1; outputs the 320 state bits after
+ *            the absorb (padding and the final p^12 included).
+ *   squeeze: inputs [0, 320) the state, then encryptions of 0 and 1; outputs
+ *            the 8 out_bytes output bits, then the 320 state bits after the
+ *            squeeze (p^12 between 8-byte blocks, none after the last; with
+ *            out_bytes <= 8 there are no gates and the outputs are inputs). */
+/* words[5] = the public XOF state after p^12 of (IV, 0, 0, 0, 0) (reference
+ * src/ascon_gpu.cu:1040-1048 DeviceASCONXOFInitialize). */
+int cufhe_ascon_xof_initial_state(uint64_t* words);
+int cufhe_circuit_ascon_absorb(int in_bytes, cufhe_circuit_sizes* sizes, cufhe_gate_desc* gates,
+                               int* layer_end, uint32_t* outputs);
+int cufhe_circuit_ascon_squeeze(int out_bytes, cufhe_circuit_sizes* sizes, cufhe_gate_desc* gates,
+                                int* layer_end, uint32_t* outputs);
 
 #ifdef __cplusplus
 }

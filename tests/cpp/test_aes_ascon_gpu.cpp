@@ -8,7 +8,8 @@
 //    threads at once (their layers share serving rounds);
 //  * ASCONXOF of the empty message gives the published Ascon v1.2 Ascon-Xof
 //    digest, and of the bytes 00 .. 0f the oracle's value
-//    (oracle/ascon_ref.py, pinned by that digest).
+//    (oracle/ascon_ref.py, pinned by that digest), and ASCONXOFInitialize /
+//    Absorb / Squeeze over an encrypted state give the same outputs.
 // Prints PASS/FAIL, returns 0/1.
 #include <array>
 #include <cstdio>
@@ -120,6 +121,28 @@ int main()
     const bool m_ok = d1 == "c861a89cfb1335f278c96cf7ffc9753c290cbe1a4e186d2923b496bb4ea5e519";
     std::printf("ASCONXOF(00..0f, 32): %s (%s)\n", d1.c_str(), m_ok ? "ok" : "WRONG");
     failures += !m_ok;
+    // the incremental form over an encrypted state: the same digests, and a
+    // one-block squeeze (no gates) gives their first 8 bytes
+    cufhe::ASCONState state;
+    cufhe::ASCONXOFInitialize(state, ek, st);
+    cufhe::ASCONXOFAbsorb(state, in, ek, st);
+    std::vector<TLWE<P>> head(64);
+    cufhe::ASCONState copy = state;
+    cufhe::ASCONXOFSqueeze(copy, head);
+    cufhe::ASCONXOFSqueeze(state, out, ek, st);
+    const std::string d2 = hex(decrypt_bytes(out.data(), out.size(), sk));
+    const std::string h2 = hex(decrypt_bytes(head.data(), head.size(), sk));
+    const bool i_ok = d2 == d1 && h2 == d1.substr(0, 16);
+    std::printf("ASCONXOFInitialize/Absorb/Squeeze(00..0f, 32): %s, first block %s (%s)\n",
+                d2.c_str(), h2.c_str(), i_ok ? "ok" : "WRONG");
+    failures += !i_ok;
+    cufhe::ASCONXOFInitialize(state);
+    cufhe::ASCONXOFAbsorb(state, empty);
+    cufhe::ASCONXOFSqueeze(state, out);
+    const std::string d3 = hex(decrypt_bytes(out.data(), out.size(), sk));
+    const bool z_ok = d3 == d0;
+    std::printf("ASCONXOFInitialize/Absorb/Squeeze(empty, 32): %s (%s)\n", d3.c_str(), z_ok ? "ok" : "WRONG");
+    failures += !z_ok;
     cufhe::CleanUpASCON();
 
     cufhe::CleanUp();

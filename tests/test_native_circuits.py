@@ -85,3 +85,34 @@ def test_native_circuit_argument_errors():
                                      out.ctypes.data) == -2
     # only some of the buffers is an argument error
     assert L.cufhe_circuit_ascon_xof(1, 8, ctypes.byref(sz), g.ctypes.data, None, None) == -1
+    # ... except the gate buffer of a circuit without gates (a one-block squeeze)
+    assert L.cufhe_circuit_ascon_squeeze(8, ctypes.byref(sz), None, None, None) == 0
+    assert sz.n_gates == 0
+    le = np.zeros(sz.n_layers, np.int32)
+    out = np.zeros(sz.n_outputs, np.uint32)
+    assert L.cufhe_circuit_ascon_squeeze(8, ctypes.byref(sz), None, le.ctypes.data,
+                                         out.ctypes.data) == 0
+    assert list(out[:64]) == [8 * (7 - j) + b for j in range(8) for b in range(8)]
+
+
+@pytest.mark.parametrize("in_bytes,out_bytes", [(0, 32), (3, 8), (16, 24)])
+def test_native_ascon_absorb_squeeze_compose_to_the_xof(in_bytes, out_bytes):
+    """Initialize (the library's public state) -> absorb -> squeeze equals
+    the oracle's XOF; the squeeze's state output is the state after its
+    permutations (the reference permutes between blocks only)."""
+    words = (ctypes.c_uint64 * 5)()
+    assert C.lib().cufhe_ascon_xof_initial_state(words) == 0
+    state = np.array([(words[i] >> k) & 1 for i in range(5) for k in range(64)], np.uint8)
+    assert list(words) == AR.permutation([AR.XOF_IV, 0, 0, 0, 0])
+    rng = np.random.default_rng(in_bytes + 1)
+    msg = bytes(rng.integers(0, 256, in_bytes, dtype=np.uint8).tolist())
+    ab = K.native_ascon_absorb(in_bytes)
+    assert ab.n_inputs == 320 + 8 * in_bytes + 2 and len(ab.outputs) == 320 and _layered(ab)
+    state = ab.evaluate(np.concatenate([state, A.bits(msg), [0, 1]]))
+    sq = K.native_ascon_squeeze(out_bytes)
+    assert sq.n_inputs == 322 and len(sq.outputs) == 8 * out_bytes + 320
+    res = sq.evaluate(np.concatenate([state, [0, 1]])) if sq.n_gates else \
+        np.concatenate([state, [0, 1]])[sq.outputs]
+    assert A.from_bits(res[:8 * out_bytes]) == AR.ascon_xof(msg, out_bytes)
+    if out_bytes <= 8:
+        assert sq.n_gates == 0 and np.array_equal(res[8 * out_bytes:], state)
