@@ -33,7 +33,10 @@ extern "C" {
 #endif
 
 #define CUFHE_AES128_INPUTS (128 + 11 * 128 + 2)
-#define CUFHE_ASCON_MAX_BYTES 4096
+/* An ASCON circuit holds every intermediate wire: about 20,000 gates (82 MB
+ * of lvl1 wires on the serving loop) per 8-byte block absorbed or squeezed
+ * beyond the first, so lengths are capped (1024 bytes: ~10 GB). */
+#define CUFHE_ASCON_MAX_BYTES 1024
 
 typedef struct {
     int n_inputs;

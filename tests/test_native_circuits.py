@@ -73,7 +73,7 @@ def test_native_circuit_argument_errors():
     sz = K._Sizes()
     assert L.cufhe_circuit_ascon_xof(-1, 8, ctypes.byref(sz), None, None, None) == -1
     assert L.cufhe_circuit_ascon_xof(8, 0, ctypes.byref(sz), None, None, None) == -1
-    assert L.cufhe_circuit_ascon_xof(5000, 8, ctypes.byref(sz), None, None, None) == -1
+    assert L.cufhe_circuit_ascon_xof(1025, 8, ctypes.byref(sz), None, None, None) == -1
     assert L.cufhe_circuit_aes128_decrypt(None, None, None, None) == -1
     # buffers smaller than the circuit are refused
     assert L.cufhe_circuit_ascon_xof(1, 8, ctypes.byref(sz), None, None, None) == 0
