@@ -41,7 +41,6 @@ same order.
 """
 import bisect
 import itertools
-import random
 
 import numpy as np
 
@@ -187,19 +186,34 @@ def _depth(levels):
     return h[0] if h else 0
 
 
+def _splitmix64(seed):
+    """Uniform doubles in [0, 1) from the splitmix64 sequence (Steele,
+    Lea and Flood 2014): the tie-breaking stream of the XOR-network search,
+    the same as the library's C++ builder uses."""
+    mask = (1 << 64) - 1
+    s = seed & mask
+    while True:
+        s = (s + 0x9E3779B97F4A7C15) & mask
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+        z ^= z >> 31
+        yield (z >> 11) * 2.0 ** -53
+
+
 def _xor_network(rows, n_in, in_level=None, tries=8):
     """XOR network computing the GF(2) rows (bit masks over n_in signals).
 
     Greedy common-subexpression search (Paar 1997): repeatedly XOR the pair
     of signals that occurs in the most rows, here only among pairs that keep
-    every row at its minimum depth; ties are broken at random and the best
-    of `tries` seeds (fewest XORs, then least depth) is kept. Returns
-    (pairs, rowsigs): pairs[t] makes signal n_in + t, rowsigs[r] lists the
-    signals XORed into row r."""
+    every row at its minimum depth; ties are broken at random (one draw per
+    candidate pair, in pair order) and the best of `tries` seeds (fewest
+    XORs, then least depth) is kept. Returns (pairs, rowsigs): pairs[t]
+    makes signal n_in + t, rowsigs[r] lists the signals XORed into row r."""
     base = list(in_level or [0] * n_in)
     best = None
     for seed in range(tries):
-        rng = random.Random(seed)
+        rng = _splitmix64(seed + 1)
         lv = list(base)
         sets = [{i for i in range(n_in) if (m >> i) & 1} for m in rows]
         bound = [_depth([lv[i] for i in s]) for s in sets]
@@ -207,13 +221,13 @@ def _xor_network(rows, n_in, in_level=None, tries=8):
         while True:
             count = {}
             for s in sets:
-                sl = sorted(s)
-                for p in itertools.combinations(sl, 2):
+                for p in itertools.combinations(sorted(s), 2):
                     count[p] = count.get(p, 0) + 1
-            cands = sorted((p for p, c in count.items() if c >= 2),
-                           key=lambda p: (-count[p], max(lv[p[0]], lv[p[1]]), rng.random()))
+            cands = [(p, count[p], max(lv[p[0]], lv[p[1]]), next(rng))
+                     for p in sorted(count) if count[p] >= 2]
+            cands.sort(key=lambda c: (-c[1], c[2], c[3]))
             chosen = None
-            for p in cands:
+            for p, _, _, _ in cands:
                 new = max(lv[p[0]], lv[p[1]]) + 1
                 if all(_depth([lv[t] for t in s if t not in p] + [new]) <= bd
                        for s, bd in zip(sets, bound) if p[0] in s and p[1] in s):

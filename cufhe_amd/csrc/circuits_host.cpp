@@ -195,7 +195,9 @@ struct Lits {
         if (hi == konst(true)) return neg(a(Lit{s, true}, neg(lo)));
         if (hi == neg(lo)) return x(Lit{s, false}, lo);
         if (hi.neg == lo.neg) return Lit{b.gate(CUFHE_MUX, lo.w, hi.w, s), hi.neg};
-        return Lit{b.gate(CUFHE_MUX, wire(lo, true), wire(hi, true), s), false};
+        const uint32_t lw = wire(lo, true);  // lo, then hi: a fixed gate order
+        const uint32_t hw = wire(hi, true);
+        return Lit{b.gate(CUFHE_MUX, lw, hw, s), false};
     }
 };
 
@@ -443,7 +445,9 @@ Lit table4(Lits& L, int tt, const std::array<uint32_t, 4>& xs, const std::array<
         if (lo == hi) return self(self, lo, k - 1);
         auto it = memo.find({tab, k});
         if (it != memo.end()) return it->second;
-        const Lit v = L.mux(xs[order[k]], self(self, hi, k - 1), self(self, lo, k - 1));
+        const Lit h = self(self, hi, k - 1);  // hi, then lo: a fixed gate order
+        const Lit l = self(self, lo, k - 1);
+        const Lit v = L.mux(xs[order[k]], h, l);
         memo.emplace(std::make_pair(tab, k), v);
         return v;
     };

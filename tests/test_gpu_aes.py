@@ -1,4 +1,4 @@
-"""AES-128 decryption circuit (cufhe_amd/aes.py, 29,568 gates, 189 layers)
+"""AES-128 decryption circuit (cufhe_amd/aes.py, 29,604 gates, 189 layers)
 run by the serving loop on encrypted inputs: the FIPS-197 Appendix C.1
 block and a random block under a random key, submitted together (one
 batch per round mixes both), decrypt to the plaintext (oracle/aes_ref.py)
