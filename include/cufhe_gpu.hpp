@@ -76,6 +76,30 @@ inline void CleanUp() { CuCheck(cufhe_cleanup(), "CleanUp"); }
 inline void CleanUp_lvl02() { CuCheck(cufhe_cleanup_lvl02(), "CleanUp_lvl02"); }
 inline void Synchronize() { CuCheck(cufhe_synchronize(), "Synchronize"); }
 
+/** Device copies of a host object, one per GPU (reference
+ *  cufhe_gpu.cuh:78-98). The host object is not page-locked: copies go
+ *  through the runtime's staging buffers (registering existing host memory
+ *  is not used by this library). */
+template <typename dT, typename hT>
+void ctxtInitialize(hT& host, std::vector<dT*>& devices)
+{
+    devices.resize(_gpuNum);
+    for (int i = 0; i < _gpuNum; i++) {
+        void* p = nullptr;
+        CuCheck(cufhe_device_alloc(&p, sizeof(host)), "ctxtInitialize");
+        devices[i] = static_cast<dT*>(p);
+    }
+}
+
+template <typename dT, typename hT>
+void ctxtDelete(hT&, std::vector<dT*>& devices)
+{
+    for (auto*& p : devices) {
+        cufhe_device_free(p);
+        p = nullptr;
+    }
+}
+
 /** Ciphertext: host TLWE plus one device copy (reference cufhe_gpu.cuh:100-121). */
 template <class P>
 struct Ctxt {

@@ -294,6 +294,24 @@ int main()
         }
         bad += report("TRGSW2NTT / CMUXNTT", e, total);
     }
+    {
+        // ctxtInitialize / ctxtDelete: device copies of a host TLWE, a
+        // round trip through them is exact
+        TLWE<lvl1param> h, back{};
+        for (uint32_t k = 0; k <= lvl1param::n; k++) h[k] = 0x9E3779B9u * (k + 1);
+        std::vector<uint32_t*> dev;
+        cufhe::ctxtInitialize(h, dev);
+        int e = dev.size() != 1 || dev[0] == nullptr;
+        if (!e) {
+            cufhe::CuCheck(cufhe_memcpy_h2d(dev[0], h.data(), sizeof(h), s.st()), "h2d");
+            cufhe::CuCheck(cufhe_memcpy_d2h(back.data(), dev[0], sizeof(h), s.st()), "d2h");
+            cufhe::Synchronize();
+            e = back != h;
+        }
+        cufhe::ctxtDelete(h, dev);
+        e += dev[0] != nullptr;
+        bad += report("ctxtInitialize / ctxtDelete", e, 1);
+    }
     s.Destroy();
     cufhe::CleanUp();
     std::printf("%s\n", bad == 0 ? "ALL PASS" : "FAIL");
