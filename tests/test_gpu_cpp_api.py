@@ -37,7 +37,8 @@ def test_cpp_launcher_layer():
     """XxxBootstrap<brP, mu, iksP> / <iksP, brP, mu> (lvl01, lvl10, lvl02),
     Mux/NMux/Copy/Not launchers, Bootstrap(out, in, mu), SEIandKS,
     BootstrapTLWE2TRLWE and IdentityKeySwitch<lvl10/lvl20> called the way
-    the reference's gate API calls them (src/cufhe_gates_gpu.cu:198-537)."""
+    the reference's gate API calls them (src/cufhe_gates_gpu.cu:198-537),
+    and a round trip through ctxtInitialize / ctxtDelete device copies."""
     p = _run("test_launchers_gpu.bin")
     assert p.returncode == 0 and "ALL PASS" in p.stdout
 
