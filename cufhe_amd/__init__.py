@@ -148,6 +148,10 @@ def lib():
         "cufhe_annihilate_key_delete": ([i32], i32),
         "cufhe_annihilate_key_switch": ([i32, vp, vp, vp], i32),
         "cufhe_annihilate_key_switch_batch": ([i32, vp, sz, vp, sz, sz, vp], i32),
+        "cufhe_bk_load": ([i32, vp, sz, i32, i32], i32),
+        "cufhe_bk_delete": ([i32], i32),
+        "cufhe_ksk_load": ([i32, vp, sz, i32], i32),
+        "cufhe_ksk_delete": ([i32], i32),
         "cufhe_circuit_aes128_decrypt": ([vp, vp, vp, vp], i32),
         "cufhe_circuit_ascon_xof": ([i32, i32, vp, vp, vp, vp], i32),
         "cufhe_circuit_ascon_absorb": ([i32, vp, vp, vp, vp], i32),

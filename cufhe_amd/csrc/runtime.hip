@@ -66,7 +66,9 @@ struct Ntt64TableSet {
 
 template <class P>
 struct LevelKeys {
-    bool loaded = false;
+    bool loaded = false;      // both keys present: the level's gates can run
+    bool bk_loaded = false;   // the bootstrapping key (bk / bk_ntt / bk_ntt64)
+    bool ksk_loaded = false;  // the key-switching key (ksk, ksg)
     uint64_t generation = 0;  // changes on every load: plans and servers check it
     bool kb = false;
     int backend = CUFHE_BACKEND_FFT;
@@ -282,6 +284,33 @@ void free_tables(TableSet<P>& ts)
     (void)hipFree(ts.tws);
     (void)hipFree(ts.e2n);
     ts = TableSet<P>{};
+}
+
+// One of a level's two keys, the other kept (BootstrappingKeyToNTT /
+// KeySwitchingKeyToDevice and their Delete* counterparts).
+template <class P>
+void free_bk(LevelKeys<P>& k)
+{
+    (void)hipFree(k.bk);
+    (void)hipFree(k.bk_ntt);
+    (void)hipFree(k.bk_ntt64);
+    k.bk = nullptr;
+    k.bk_ntt = nullptr;
+    k.bk_ntt64 = nullptr;
+    k.fk = FourierKeys<P>{};
+    k.nk = NttKeys{};
+    k.n64 = Ntt64Keys{};
+    k.bk_loaded = false;
+    k.loaded = false;
+}
+template <class P>
+void free_ksk(LevelKeys<P>& k)
+{
+    (void)hipFree(k.ksk);
+    k.ksk = nullptr;
+    ks_gemm_release(k.ksg);
+    k.ksk_loaded = false;
+    k.loaded = false;
 }
 
 template <class P>
@@ -550,6 +579,71 @@ int load_ksk(uint32_t** dst, const uint32_t* ksk, size_t words, size_t want)
     CUFHE_HIP_TRY(hipMalloc(reinterpret_cast<void**>(dst), words * sizeof(uint32_t)));
     CUFHE_HIP_TRY(hipMemcpy(*dst, ksk, words * sizeof(uint32_t), hipMemcpyHostToDevice));
     return CUFHE_OK;
+}
+
+// One key of a level into the device state, the other key kept; the
+// caller holds the state lock with the device idle. On failure the key is
+// absent. The level's gates run once both keys are present (note_load).
+int bk_load_lvl1(State& s, const uint32_t* bk, size_t words, int backend, int variant)
+{
+    free_bk(s.k1);
+    const int rc = backend == CUFHE_BACKEND_NTT
+                       ? load_bk_ntt(s.k1, s.tn, bk, words, variant)
+                       : load_bk<TFHEpp::lvl01param>(s.k1, s.t1, bk, words,
+                                                     variant == CUFHE_BR_KEY_BUNDLE);
+    if (rc != CUFHE_OK) {
+        free_bk(s.k1);
+        return rc;
+    }
+    s.k1.backend = backend;
+    s.k1.variant = variant;
+    s.k1.kb = variant == CUFHE_BR_KEY_BUNDLE;
+    s.k1.bk_loaded = true;
+    return CUFHE_OK;
+}
+int bk_load_lvl2(State& s, const uint64_t* bk, size_t words, int backend, bool kb)
+{
+    free_bk(s.k2);
+    const int rc = backend == CUFHE_BACKEND_NTT ? load_bk_ntt64(s.k2, s.t64, bk, words, kb)
+                                                : load_bk<TFHEpp::lvl02param>(s.k2, s.t2, bk, words, kb);
+    if (rc != CUFHE_OK) {
+        free_bk(s.k2);
+        return rc;
+    }
+    s.k2.backend = backend;
+    s.k2.kb = kb;
+    s.k2.bk_loaded = true;
+    return CUFHE_OK;
+}
+template <class P>
+int ksk_load(LevelKeys<P>& k, int level, const uint32_t* ksk, size_t words, bool subset)
+{
+    free_ksk(k);
+    size_t want = TFHEpp::kKSKWords<TFHEpp::lvl20param>;
+    if (level == 1) {
+        using KS = TFHEpp::lvl10param;
+        want = subset ? static_cast<size_t>(KS::domainP::n - KS::targetP::n) * KS::t *
+                            (1u << (KS::basebit - 1)) * (KS::targetP::n + 1)
+                      : TFHEpp::kKSKWords<KS>;
+    }
+    int rc = load_ksk(&k.ksk, ksk, words, want);
+    if (rc == CUFHE_OK && ks_gemm_prepare(k.ksg, k.ksk, level, subset, nullptr) != 0)
+        rc = fail(CUFHE_ERR_HIP, "key-switching key byte planes");
+    if (rc != CUFHE_OK) {
+        free_ksk(k);
+        return rc;
+    }
+    k.subset = subset;
+    k.ksk_loaded = true;
+    return CUFHE_OK;
+}
+// A key of the level changed: plans and servers built before see a new
+// generation.
+template <class P>
+void note_load(State& s, LevelKeys<P>& k)
+{
+    k.loaded = k.bk_loaded && k.ksk_loaded;
+    k.generation = ++s.generations;
 }
 
 // Linear combination of a binary gate (reference src/bootstrap_gpu.cu
@@ -937,28 +1031,13 @@ int cufhe_initialize_lvl01_ex(const uint32_t* bk, size_t bk_words, int backend, 
     if (rc != CUFHE_OK) return rc;
     (void)hipDeviceSynchronize();
     free_keys(s.k1);
-    if (backend == CUFHE_BACKEND_NTT)
-        rc = load_bk_ntt(s.k1, s.tn, bk, bk_words, variant);
-    else
-        rc = load_bk<TFHEpp::lvl01param>(s.k1, s.t1, bk, bk_words, variant == CUFHE_BR_KEY_BUNDLE);
-    using KS = TFHEpp::lvl10param;
-    const size_t ksk_want =
-        subset_ksk ? static_cast<size_t>(KS::domainP::n - KS::targetP::n) * KS::t *
-                         (1u << (KS::basebit - 1)) * (KS::targetP::n + 1)
-                   : TFHEpp::kKSKWords<KS>;
-    if (rc == CUFHE_OK) rc = load_ksk(&s.k1.ksk, ksk, ksk_words, ksk_want);
-    if (rc == CUFHE_OK && ks_gemm_prepare(s.k1.ksg, s.k1.ksk, 1, subset_ksk != 0, nullptr) != 0)
-        rc = fail(CUFHE_ERR_HIP, "key-switching key byte planes");
+    rc = bk_load_lvl1(s, bk, bk_words, backend, variant);
+    if (rc == CUFHE_OK) rc = ksk_load(s.k1, 1, ksk, ksk_words, subset_ksk != 0);
     if (rc != CUFHE_OK) {
         free_keys(s.k1);
         return rc;
     }
-    s.k1.backend = backend;
-    s.k1.variant = variant;
-    s.k1.kb = variant == CUFHE_BR_KEY_BUNDLE;
-    s.k1.subset = subset_ksk != 0;
-    s.k1.generation = ++s.generations;
-    s.k1.loaded = true;
+    note_load(s, s.k1);
     return CUFHE_OK;
 }
 
@@ -978,21 +1057,95 @@ int cufhe_initialize_lvl02_ex(const uint64_t* bk, size_t bk_words, int backend, 
     std::lock_guard<std::mutex> g(s.mu);
     int rc = ensure_init();
     if (rc != CUFHE_OK) return rc;
+    (void)hipDeviceSynchronize();
     free_keys(s.k2);
-    if (backend == CUFHE_BACKEND_NTT)
-        rc = load_bk_ntt64(s.k2, s.t64, bk, bk_words, key_bundle != 0);
-    else
-        rc = load_bk<TFHEpp::lvl02param>(s.k2, s.t2, bk, bk_words, key_bundle != 0);
-    if (rc == CUFHE_OK) rc = load_ksk(&s.k2.ksk, ksk, ksk_words, TFHEpp::kKSKWords<TFHEpp::lvl20param>);
-    if (rc == CUFHE_OK && ks_gemm_prepare(s.k2.ksg, s.k2.ksk, 2, false, nullptr) != 0)
-        rc = fail(CUFHE_ERR_HIP, "key-switching key byte planes");
+    rc = bk_load_lvl2(s, bk, bk_words, backend, key_bundle != 0);
+    if (rc == CUFHE_OK) rc = ksk_load(s.k2, 2, ksk, ksk_words, false);
     if (rc != CUFHE_OK) {
         free_keys(s.k2);
         return rc;
     }
-    s.k2.backend = backend;
-    s.k2.generation = ++s.generations;
-    s.k2.loaded = true;
+    note_load(s, s.k2);
+    return CUFHE_OK;
+}
+
+int cufhe_bk_load(int level, const void* bk, size_t words, int backend, int variant)
+{
+    if (!bk) return fail(CUFHE_ERR_ARG, "null key pointer");
+    if (level != 1 && level != 2) return fail(CUFHE_ERR_ARG, "level must be 1 or 2");
+    if (backend != CUFHE_BACKEND_FFT && backend != CUFHE_BACKEND_NTT)
+        return fail(CUFHE_ERR_ARG, "unknown backend " + std::to_string(backend));
+    if (variant < CUFHE_BR_PLAIN || variant > CUFHE_BR_BLOCK_BINARY)
+        return fail(CUFHE_ERR_ARG, "unknown blind-rotation variant " + std::to_string(variant));
+    if (variant == CUFHE_BR_BLOCK_BINARY && (level == 2 || backend == CUFHE_BACKEND_FFT))
+        return fail(CUFHE_ERR_UNSUPPORTED, "block-binary blind rotation is lvl1 on the NTT backend");
+    State& s = state();
+    std::lock_guard<std::mutex> g(s.mu);
+    int rc = ensure_init();
+    if (rc != CUFHE_OK) return rc;
+    (void)hipDeviceSynchronize();
+    if (level == 1) {
+        rc = bk_load_lvl1(s, static_cast<const uint32_t*>(bk), words, backend, variant);
+        note_load(s, s.k1);
+    } else {
+        rc = bk_load_lvl2(s, static_cast<const uint64_t*>(bk), words, backend,
+                          variant == CUFHE_BR_KEY_BUNDLE);
+        note_load(s, s.k2);
+    }
+    return rc;
+}
+
+int cufhe_bk_delete(int level)
+{
+    if (level != 1 && level != 2) return fail(CUFHE_ERR_ARG, "level must be 1 or 2");
+    State& s = state();
+    std::lock_guard<std::mutex> g(s.mu);
+    if (!s.initialized) return CUFHE_OK;
+    (void)hipDeviceSynchronize();
+    if (level == 1) {
+        free_bk(s.k1);
+        note_load(s, s.k1);
+    } else {
+        free_bk(s.k2);
+        note_load(s, s.k2);
+    }
+    return CUFHE_OK;
+}
+
+int cufhe_ksk_load(int level, const uint32_t* ksk, size_t words, int subset)
+{
+    if (!ksk) return fail(CUFHE_ERR_ARG, "null key pointer");
+    if (level != 1 && level != 2) return fail(CUFHE_ERR_ARG, "level must be 1 or 2");
+    if (level == 2 && subset) return fail(CUFHE_ERR_UNSUPPORTED, "the subset key is lvl10 only");
+    State& s = state();
+    std::lock_guard<std::mutex> g(s.mu);
+    int rc = ensure_init();
+    if (rc != CUFHE_OK) return rc;
+    (void)hipDeviceSynchronize();
+    if (level == 1) {
+        rc = ksk_load(s.k1, 1, ksk, words, subset != 0);
+        note_load(s, s.k1);
+    } else {
+        rc = ksk_load(s.k2, 2, ksk, words, false);
+        note_load(s, s.k2);
+    }
+    return rc;
+}
+
+int cufhe_ksk_delete(int level)
+{
+    if (level != 1 && level != 2) return fail(CUFHE_ERR_ARG, "level must be 1 or 2");
+    State& s = state();
+    std::lock_guard<std::mutex> g(s.mu);
+    if (!s.initialized) return CUFHE_OK;
+    (void)hipDeviceSynchronize();
+    if (level == 1) {
+        free_ksk(s.k1);
+        note_load(s, s.k1);
+    } else {
+        free_ksk(s.k2);
+        note_load(s, s.k2);
+    }
     return CUFHE_OK;
 }
 

@@ -112,6 +112,22 @@ int cufhe_initialize_lvl02(const uint64_t* bk, size_t bk_words, int key_bundle,
  * cufhe_initialize_lvl02, which is this call with CUFHE_BACKEND_FFT. */
 int cufhe_initialize_lvl02_ex(const uint64_t* bk, size_t bk_words, int backend, int key_bundle,
                               const uint32_t* ksk, size_t ksk_words);
+/* One key of a level at a time, the other kept (reference
+ * include/bootstrap_gpu.cuh:43-51 BootstrappingKeyToNTT /
+ * BootstrappingKeyBundleToNTT / DeleteBootstrappingKeyNTT[_lvl02],
+ * include/keyswitch_gpu.cuh:258-271 KeySwitchingKeyToDevice[_lvl20] /
+ * DeleteKeySwitchingKey[_lvl20]). level 1: bk is BootstrappingKey<lvl01param>
+ * (uint32 words; variant CUFHE_BR_PLAIN, CUFHE_BR_KEY_BUNDLE, or
+ * CUFHE_BR_BLOCK_BINARY on the NTT backend) and ksk KeySwitchingKey<lvl10param>
+ * (or the subset key with subset != 0); level 2: bk is
+ * BootstrappingKey<lvl02param> (uint64 words; CUFHE_BR_PLAIN or
+ * CUFHE_BR_KEY_BUNDLE) and ksk KeySwitchingKey<lvl20param>. A level's gates
+ * run once both of its keys are loaded; every load or delete counts as a
+ * key change for plans and servers built before. */
+int cufhe_bk_load(int level, const void* bk, size_t words, int backend, int variant);
+int cufhe_bk_delete(int level);
+int cufhe_ksk_load(int level, const uint32_t* ksk, size_t words, int subset);
+int cufhe_ksk_delete(int level);
 /* CleanUp() / CleanUp_lvl02(). */
 int cufhe_cleanup(void);
 int cufhe_cleanup_lvl02(void);

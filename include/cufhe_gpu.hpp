@@ -76,6 +76,65 @@ inline void CleanUp() { CuCheck(cufhe_cleanup(), "CleanUp"); }
 inline void CleanUp_lvl02() { CuCheck(cufhe_cleanup_lvl02(), "CleanUp_lvl02"); }
 inline void Synchronize() { CuCheck(cufhe_synchronize(), "Synchronize"); }
 
+/** One key of a level at a time, the other kept (reference
+ *  include/bootstrap_gpu.cuh:40-51, include/keyswitch_gpu.cuh:258-271); the
+ *  level's gates run once both are loaded. The bootstrapping key's layout
+ *  (one TRGSW per lvl0 key bit, or the 2-bit bundle) follows from its size;
+ *  backend: CUFHE_BACKEND_FFT (the reference's default build) or
+ *  CUFHE_BACKEND_NTT; block_binary: the block-binary rotation (lvl01param,
+ *  NTT). The subset lvl10 key is recognised by its size. gpuNum is the
+ *  reference's GPU count (one GPU per process here). */
+inline void InitializeNTThandlers(const int) { CuCheck(cufhe_initialize(), "InitializeNTThandlers"); }
+inline void InitializeNTThandlers_lvl02(const int)
+{
+    CuCheck(cufhe_initialize(), "InitializeNTThandlers_lvl02");
+}
+template <class brP>
+void BootstrappingKeyToNTT(const TFHEpp::BootstrappingKey<brP>& bk, const int,
+                           const int backend = CUFHE_BACKEND_FFT, const bool block_binary = false)
+{
+    static_assert(std::is_same_v<brP, TFHEpp::lvl01param> || std::is_same_v<brP, TFHEpp::lvl02param>,
+                  "bootstrapping keys: lvl01param, lvl02param");
+    constexpr int level = std::is_same_v<brP, TFHEpp::lvl01param> ? 1 : 2;
+    const bool kb = bk.size() == tfhe_bk_words(level, 1);
+    const int variant = block_binary ? CUFHE_BR_BLOCK_BINARY
+                                     : (kb ? CUFHE_BR_KEY_BUNDLE : CUFHE_BR_PLAIN);
+    CuCheck(cufhe_bk_load(level, bk.data(), bk.size(), block_binary ? CUFHE_BACKEND_NTT : backend,
+                          variant),
+            "BootstrappingKeyToNTT");
+}
+template <class brP>
+void BootstrappingKeyBundleToNTT(const TFHEpp::BootstrappingKey<brP>& bk, const int gpuNum,
+                                 const int backend = CUFHE_BACKEND_FFT)
+{
+    constexpr int level = std::is_same_v<brP, TFHEpp::lvl01param> ? 1 : 2;
+    if (bk.size() != tfhe_bk_words(level, 1)) {
+        std::fprintf(stderr, "cufhe: BootstrappingKeyBundleToNTT: not a key-bundle key\n");
+        std::exit(-1);
+    }
+    BootstrappingKeyToNTT<brP>(bk, gpuNum, backend);
+}
+inline void DeleteBootstrappingKeyNTT(const int) { CuCheck(cufhe_bk_delete(1), "DeleteBootstrappingKeyNTT"); }
+inline void DeleteBootstrappingKeyNTT_lvl02(const int)
+{
+    CuCheck(cufhe_bk_delete(2), "DeleteBootstrappingKeyNTT_lvl02");
+}
+inline void KeySwitchingKeyToDevice(const TFHEpp::KeySwitchingKey<TFHEpp::lvl10param>& ksk, const int)
+{
+    const int subset = ksk.size() == tfhe_ksk_subset_words() ? 1 : 0;
+    CuCheck(cufhe_ksk_load(1, ksk.data(), ksk.size(), subset), "KeySwitchingKeyToDevice");
+}
+inline void DeleteKeySwitchingKey(const int) { CuCheck(cufhe_ksk_delete(1), "DeleteKeySwitchingKey"); }
+inline void KeySwitchingKeyToDevice_lvl20(const TFHEpp::KeySwitchingKey<TFHEpp::lvl20param>& ksk,
+                                          const int)
+{
+    CuCheck(cufhe_ksk_load(2, ksk.data(), ksk.size(), 0), "KeySwitchingKeyToDevice_lvl20");
+}
+inline void DeleteKeySwitchingKey_lvl20(const int)
+{
+    CuCheck(cufhe_ksk_delete(2), "DeleteKeySwitchingKey_lvl20");
+}
+
 /** Device copies of a host object, one per GPU (reference
  *  cufhe_gpu.cuh:78-98). The host object is not page-locked: copies go
  *  through the runtime's staging buffers (registering existing host memory

@@ -99,6 +99,14 @@ struct SecretKey {
     }
 };
 
+// Bootstrapping key of brP in the polynomial domain (TRGSW after TRGSW:
+// one per lvl0 key bit, or three per bit pair in the key-bundle layout) and
+// key-switching key of ksP, as EvalKey holds them.
+template <class brP>
+using BootstrappingKey = std::vector<typename brP::targetP::T>;
+template <class ksP>
+using KeySwitchingKey = std::vector<uint32_t>;
+
 // Bootstrapping keys (polynomial domain) and key-switching keys.
 struct EvalKey {
     // Key files (tfhe_keyfile_save/_load, tfhe_host.h): the present arrays
@@ -152,6 +160,28 @@ struct EvalKey {
         return (key_bundle ? 1u : 0u) | (block_binary ? 2u : 0u) | (subset_iksk ? 4u : 0u) |
                (ntt ? 8u : 0u);
     }
+    // getbk<lvl01param / lvl02param>(), getiksk<lvl10param / lvl20param>()
+    template <class brP>
+    const BootstrappingKey<brP>& getbk() const
+    {
+        static_assert(std::is_same_v<brP, lvl01param> || std::is_same_v<brP, lvl02param>,
+                      "bootstrapping keys: lvl01param, lvl02param");
+        if constexpr (std::is_same_v<brP, lvl01param>)
+            return *bklvl01;
+        else
+            return *bklvl02;
+    }
+    template <class ksP>
+    const KeySwitchingKey<ksP>& getiksk() const
+    {
+        static_assert(std::is_same_v<ksP, lvl10param> || std::is_same_v<ksP, lvl20param>,
+                      "key-switching keys: lvl10param, lvl20param");
+        if constexpr (std::is_same_v<ksP, lvl10param>)
+            return *iksklvl10;
+        else
+            return *iksklvl20;
+    }
+
     bool key_bundle = true;    // 2-bit key bundle (reference default build)
     bool block_binary = false;  // block-binary rotation (NTT backend; reference USE_BLOCK_BINARY)
     bool subset_iksk = false;   // iksklvl10 holds the subset key (emplacesubiksk)

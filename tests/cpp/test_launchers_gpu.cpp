@@ -295,6 +295,39 @@ int main()
         bad += report("TRGSW2NTT / CMUXNTT", e, total);
     }
     {
+        // BootstrappingKeyToNTT / KeySwitchingKeyToDevice and the Delete*
+        // functions: one key at a time; gates are refused until a level has
+        // both keys, then run correctly
+        int e = 0, total = 0;
+        auto c0 = tlweSymEncrypt<lvl0param>(lvl0param::μ, lvl0param::α, sk.key.lvl0);
+        auto c1 = tlweSymEncrypt<lvl0param>(-lvl0param::μ, lvl0param::α, sk.key.lvl0);
+        uint32_t *d0 = upload<lvl0param>(c0), *d1 = upload<lvl0param>(c1);
+        uint32_t* o = dev_alloc<lvl0param>(lvl0param::n + 1);
+        cufhe::DeleteKeySwitchingKey(1);
+        e += cufhe_gate_lvl0(CUFHE_NAND, o, d0, d1, nullptr, st) != CUFHE_ERR_STATE;
+        cufhe::DeleteBootstrappingKeyNTT(1);
+        cufhe::BootstrappingKeyToNTT<lvl01param>(ek.getbk<lvl01param>(), 1);
+        e += cufhe_gate_lvl0(CUFHE_NAND, o, d0, d1, nullptr, st) != CUFHE_ERR_STATE;
+        cufhe::KeySwitchingKeyToDevice(ek.getiksk<lvl10param>(), 1);
+        total += 2;
+        cufhe::DeleteKeySwitchingKey_lvl20(1);
+        cufhe::DeleteBootstrappingKeyNTT_lvl02(1);
+        cufhe::KeySwitchingKeyToDevice_lvl20(ek.getiksk<lvl20param>(), 1);
+        cufhe::BootstrappingKeyToNTT<lvl02param>(ek.getbk<lvl02param>(), 1);
+        for (int x = 0; x < 2; x++)
+            for (int y = 0; y < 2; y++) {
+                uint32_t* a = x ? d0 : d1;
+                uint32_t* b = y ? d0 : d1;
+                cufhe::NandBootstrap<lvl01param, mu1, lvl10param>(o, a, b, st, 0);
+                e += tlweSymDecrypt<lvl0param>(download<lvl0param>(o), sk.key.lvl0) != (1 - x * y);
+                cufhe::NandBootstrap<lvl02param, mu2, lvl20param>(o, a, b, st, 0);
+                e += tlweSymDecrypt<lvl0param>(download<lvl0param>(o), sk.key.lvl0) != (1 - x * y);
+                total += 2;
+            }
+        for (uint32_t* p : {d0, d1, o}) cufhe_device_free(p);
+        bad += report("BootstrappingKeyToNTT / KeySwitchingKeyToDevice", e, total);
+    }
+    {
         // ctxtInitialize / ctxtDelete: device copies of a host TLWE, a
         // round trip through them is exact
         TLWE<lvl1param> h, back{};

@@ -41,6 +41,24 @@ def test_version_and_error_paths_without_gpu_work():
     assert lib.cufhe_plan_create_lvl1(None, 0, None, 0, 0, ctypes.byref(ctypes.c_void_p())) == -3
 
 
+def test_separate_key_loading_argument_errors_without_gpu_work():
+    """cufhe_bk_load / cufhe_ksk_load / *_delete (BootstrappingKeyToNTT,
+    KeySwitchingKeyToDevice ...) refuse bad arguments before touching a GPU."""
+    lib = C.lib()
+    buf = np.zeros(16, np.uint64)
+    p = buf.ctypes.data
+    assert lib.cufhe_bk_load(3, p, 16, 0, 1) == -1            # level
+    assert lib.cufhe_bk_load(1, None, 16, 0, 1) == -1         # null key
+    assert lib.cufhe_bk_load(1, p, 16, 2, 1) == -1            # backend
+    assert lib.cufhe_bk_load(1, p, 16, 0, 3) == -1            # variant
+    assert lib.cufhe_bk_load(1, p, 16, 0, 2) == -5            # block-binary needs NTT
+    assert lib.cufhe_bk_load(2, p, 16, 1, 2) == -5            # ... and lvl1
+    assert lib.cufhe_ksk_load(0, p, 16, 0) == -1
+    assert lib.cufhe_ksk_load(1, None, 16, 0) == -1
+    assert lib.cufhe_ksk_load(2, p, 16, 1) == -5              # subset key is lvl10 only
+    assert lib.cufhe_bk_delete(0) == -1 and lib.cufhe_ksk_delete(5) == -1
+
+
 def test_annihilation_argument_errors_without_gpu_work():
     """Checked before any device call: bad level, null pointers, strides below
     the TRLWE size, an NTT key at level 1, a key of the wrong size (host
