@@ -591,6 +591,14 @@ inline void gCMUXNTT(cuFHETRLWElvl1& res, cuFHETRGSWNTTlvl1& cs, cuFHETRLWElvl1&
                             c0.trlwedevices[0], 1, st.st()),
             "CMUXNTT");
 }
+/** CMUXNTTkernel (reference src/bootstrap_gpu.cu:2357-2369): res = c0 +
+ *  ExtProd(c1 - c0, cs) on device pointers, cs a TRGSW in the loaded
+ *  backend's transform domain (TRGSW2NTT's device copy). */
+inline void CMUXNTTkernel(uint32_t* const res, const void* const cs, uint32_t* const c1,
+                          uint32_t* const c0, hipStream_t st, const int)
+{
+    CuCheck(cufhe_cmux_lvl1(res, cs, c1, c0, 1, st), "CMUXNTTkernel");
+}
 inline void CMUXNTT(cuFHETRLWElvl1& res, cuFHETRGSWNTTlvl1& cs, cuFHETRLWElvl1& c1,
                     cuFHETRLWElvl1& c0, Stream st)
 {

@@ -264,7 +264,8 @@ int main()
     }
     // TRGSW2NTT / CMUXNTT (reference src/cufhe_gates_gpu.cu:116-133,
     // src/bootstrap_gpu.cu:165-186): the result decrypts to c1's message when
-    // the TRGSW encrypts 1 and to c0's when it encrypts 0
+    // the TRGSW encrypts 1 and to c0's when it encrypts 0; CMUXNTTkernel on
+    // the same device operands gives the same TRLWE
     {
         int e = 0, total = 0;
         for (int i = 0; i < 6; i++) {
@@ -283,6 +284,15 @@ int main()
             trlweSymEncrypt(c0.trlwehost, p0, sk.key.lvl1);
             cufhe::CMUXNTT(res, cs, c1, c0, s);
             cufhe::Synchronize();
+            // the raw-pointer form on the same device operands gives the
+            // same TRLWE
+            cufhe::cuFHETRLWElvl1 raw;
+            cufhe::CMUXNTTkernel(raw.trlwedevices[0], cs.trgswdevices[0], c1.trlwedevices[0],
+                                 c0.trlwedevices[0], st, 0);
+            cufhe::CuCheck(cufhe_memcpy_d2h(raw.trlwehost.data(), raw.trlwedevices[0],
+                                            sizeof(raw.trlwehost), st), "d2h");
+            cufhe::Synchronize();
+            e += raw.trlwehost != res.trlwehost;
             // SampleExtract of coefficient 0 on the host
             TLWE<lvl1param> t;
             t[0] = res.trlwehost[0][0];
