@@ -114,6 +114,17 @@ void BootstrappingKeyBundleToNTT(const TFHEpp::BootstrappingKey<brP>& bk, const 
     }
     BootstrappingKeyToNTT<brP>(bk, gpuNum, backend);
 }
+/** BootstrappingKeyFlatToNTT_lvl02 (reference src/bootstrap_gpu.cu:382-452):
+ *  the lvl02 bootstrapping key as num_elements TRGSWs in a row (n of them,
+ *  or 3 n / 2 in the key-bundle layout). */
+inline void BootstrappingKeyFlatToNTT_lvl02(const TFHEpp::TRGSW<TFHEpp::lvl2param>* flat_bk,
+                                            const uint32_t num_elements, const int,
+                                            const int backend = CUFHE_BACKEND_FFT)
+{
+    const size_t words = static_cast<size_t>(num_elements) * TFHEpp::kTRGSWWords<TFHEpp::lvl2param>;
+    const int variant = words == tfhe_bk_words(2, 1) ? CUFHE_BR_KEY_BUNDLE : CUFHE_BR_PLAIN;
+    CuCheck(cufhe_bk_load(2, flat_bk, words, backend, variant), "BootstrappingKeyFlatToNTT_lvl02");
+}
 inline void DeleteBootstrappingKeyNTT(const int) { CuCheck(cufhe_bk_delete(1), "DeleteBootstrappingKeyNTT"); }
 inline void DeleteBootstrappingKeyNTT_lvl02(const int)
 {

@@ -305,9 +305,10 @@ int main()
         bad += report("TRGSW2NTT / CMUXNTT", e, total);
     }
     {
-        // BootstrappingKeyToNTT / KeySwitchingKeyToDevice and the Delete*
-        // functions: one key at a time; gates are refused until a level has
-        // both keys, then run correctly
+        // BootstrappingKeyToNTT (and the flat lvl02 form) /
+        // KeySwitchingKeyToDevice and the Delete* functions: one key at a
+        // time; gates are refused until a level has both keys, then run
+        // correctly
         int e = 0, total = 0;
         auto c0 = tlweSymEncrypt<lvl0param>(lvl0param::μ, lvl0param::α, sk.key.lvl0);
         auto c1 = tlweSymEncrypt<lvl0param>(-lvl0param::μ, lvl0param::α, sk.key.lvl0);
@@ -324,6 +325,11 @@ int main()
         cufhe::DeleteBootstrappingKeyNTT_lvl02(1);
         cufhe::KeySwitchingKeyToDevice_lvl20(ek.getiksk<lvl20param>(), 1);
         cufhe::BootstrappingKeyToNTT<lvl02param>(ek.getbk<lvl02param>(), 1);
+        // the same key as a flat TRGSW array
+        static_assert(sizeof(TRGSW<lvl2param>) == kTRGSWWords<lvl2param> * sizeof(uint64_t));
+        const auto& bk2 = ek.getbk<lvl02param>();
+        cufhe::BootstrappingKeyFlatToNTT_lvl02(reinterpret_cast<const TRGSW<lvl2param>*>(bk2.data()),
+                                               static_cast<uint32_t>(bk2.size() / kTRGSWWords<lvl2param>), 1);
         for (int x = 0; x < 2; x++)
             for (int y = 0; y < 2; y++) {
                 uint32_t* a = x ? d0 : d1;
