@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "cufhe_amd.h"
+#include "error_gpu.hpp"
 #include "tfhe_host.hpp"
 
 namespace cufhe {

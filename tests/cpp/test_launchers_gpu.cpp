@@ -359,6 +359,8 @@ int main()
         }
         cufhe::ctxtDelete(h, dev);
         e += dev[0] != nullptr;
+        CuSafeCall(hipStreamSynchronize(st));  // the reference's error helpers
+        CuCheckError();
         bad += report("ctxtInitialize / ctxtDelete", e, 1);
     }
     s.Destroy();
